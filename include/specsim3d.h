@@ -1,0 +1,114 @@
+/*
+ * specsim3d.h -- C ABI of the MI355X spectral-simulation engine (libspecsim3d.so).
+ *
+ * The reference program (src/specsim3d.f90) has no in-process API: its hot path is the inline
+ * block of the main program f90:1234-1345 calling module sim3d_mod (f90:89-304).  Each entry
+ * point below replaces one piece of that block; the Fortran driver (specsim3d_amd/fortran)
+ * binds them with ISO_C_BINDING and keeps the reference's parameter file, stdout and output
+ * files.
+ *
+ * Conventions
+ *  - Plain C types only.  Every function returns SS3D_OK (0) or a positive SS3D_E* code; the
+ *    message of the last failure on the calling thread is available from ss3d_last_error().
+ *  - Arrays follow the reference's Fortran layout: x fastest, then y, then z, then realization.
+ *  - The library owns device memory (spectra, work buffers); callers own host arrays.  Entry
+ *    points that take device pointers say so; those pointers must come from hipMalloc on the
+ *    context's device.
+ *  - A context is used by one host thread at a time; work is ordered on the context's stream.
+ */
+#ifndef SPECSIM3D_H
+#define SPECSIM3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS3D_OK 0
+#define SS3D_EINVAL 1   /* invalid argument (message says which) */
+#define SS3D_ETOOBIG 2  /* grid or axis beyond what the engine supports */
+#define SS3D_ENOMEM 3   /* device or host allocation failed */
+#define SS3D_EHIP 4     /* HIP runtime error */
+#define SS3D_ENODEV 5   /* no usable GPU */
+
+/* arithmetic of the per-realization path */
+#define SS3D_FP64 0 /* drop-in: C2C x->y->z in FP64, reference butterfly order (f90:1335) */
+#define SS3D_FP32 1 /* fast: FP32 half-spectrum z->y passes + C2R x pass, FP64 statistics */
+
+/* One nested structure: the scalar members of `type structure` (f90:95-100).  The rotation
+ * (setrot, f90:107-139) is recomputed inside the library.  Angles in degrees. */
+typedef struct ss3d_structure {
+    int32_t itype; /* 1 spherical, 2 exponential, 3 gaussian */
+    int32_t pad_;
+    double sill, ahmax, ahmin, avert, az, dip, plg;
+} ss3d_structure;
+
+/* Spectral-density diagnostics: the values the reference prints at f90:1289-1292. */
+typedef struct ss3d_spec_diag {
+    double max_imag; /* max |Im S| */
+    double max_real; /* max |Re S| */
+    double c0;       /* C(0), the covariance at lag 0 (f90:1270) */
+    double spos;     /* sum of the non-negative Re S */
+    double sneg;     /* sum of the negative Re S (clipped to zero amplitude) */
+} ss3d_spec_diag;
+
+typedef struct ss3d_context ss3d_context;
+typedef struct ss3d_spectrum ss3d_spectrum;
+
+int ss3d_version(void);
+
+/* Copies the last error message of the calling thread (NUL-terminated, truncated to len). */
+int ss3d_last_error(char* buf, int64_t len);
+
+/* Opens a context on HIP device `device` (stream, twiddle tables, work-buffer pool). */
+int ss3d_context_create(int device, ss3d_context** out);
+int ss3d_context_destroy(ss3d_context* ctx);
+int ss3d_context_synchronize(ss3d_context* ctx);
+
+/* Periodic grid size (replaces f90:1234-1252 with cutoff f90:142 and nextpow2 f90:181).
+ * ipad=0: next power of two >= n; ipad=1: nextpow2(max(n+r, 2r)) with r the covariance reach
+ * in cells; nz=1 keeps npz=1.  Unlike the reference (f90:1253) no 2^27 cap is applied here:
+ * the caller decides (the Fortran driver keeps the reference's cap unless told otherwise). */
+int ss3d_periodic_grid(const ss3d_structure* st, int nst, double dx, double dy, double dz, int nx, int ny,
+                       int nz, int ipad, int* npx, int* npy, int* npz);
+
+/* Spectral density on the device (replaces f90:1256-1288 with covariance f90:153 and the
+ * forward fft3d f90:254): covariance sampled on the periodic grid with minimum-image lags,
+ * forward FP64 transform, S = Re/N, amplitude sqrt(max(S,0)).  Keeps the FP64 amplitude (for
+ * SS3D_FP64 realizations) and the FP32 half-spectrum amplitude (for SS3D_FP32).  The handle
+ * caches the random-stream tables of the grid.  diag may be NULL. */
+int ss3d_spectrum_create(ss3d_context* ctx, const ss3d_structure* st, int nst, double nugget, double dx,
+                         double dy, double dz, int npx, int npy, int npz, ss3d_spectrum** out,
+                         ss3d_spec_diag* diag);
+int ss3d_spectrum_destroy(ss3d_spectrum* spec);
+/* Device bytes held by a spectrum handle. */
+int64_t ss3d_spectrum_bytes(const ss3d_spectrum* spec);
+/* Copies the FP64 amplitude sqrt(max(S,0)) (npx*npy*npz doubles, x fastest) to host memory. */
+int ss3d_spectrum_amplitude(const ss3d_spectrum* spec, double* host_amp);
+
+/* Random stream bookkeeping (f90:191-195, 1302-1303, fill order f90:1305-1334).
+ * draws per realization: (N + n_sc)/2 points draw once (igauss=0) or twice (igauss=1). */
+int64_t ss3d_draws_per_realization(int npx, int npy, int npz, int igauss);
+/* Park-Miller state that realization r (0-based) starts from, seeded as f90:1302-1303. */
+int64_t ss3d_realization_state(int iseed, int npx, int npy, int npz, int igauss, int64_t r);
+
+/* Realizations r0 .. r0+nreal-1 of the stream seeded by iseed (replaces f90:1305-1344):
+ * Hermitian coefficient fill, inverse transform, crop of the first nx*ny*nz corner.
+ * host_out: nx*ny*nz*nreal values, double for SS3D_FP64 and float for SS3D_FP32.
+ * mean, var, maximag: nreal doubles each (may be NULL) -- the per-realization statistics the
+ * reference prints at f90:1343 (population variance, max |Im| of the cropped corner). */
+int ss3d_realize(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int igauss, int64_t r0, int nreal,
+                 int nx, int ny, int nz, int precision, void* host_out, double* mean, double* var,
+                 double* maximag);
+
+/* Same work with the output left in device memory and no host synchronisation: dev_out holds
+ * nx*ny*nz*nreal values (double or float by precision), dev_stats 3*nreal doubles
+ * (mean, variance, max|Im| per realization) or NULL.  Ordered on the context stream. */
+int ss3d_realize_device(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int igauss, int64_t r0,
+                        int nreal, int nx, int ny, int nz, int precision, void* dev_out, double* dev_stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""specsim3d_amd -- MI355X engine for SPECSIM3D's unconditional spectral simulation.
+
+The product is native: libspecsim3d.so (HIP kernels for gfx950 behind the C ABI of
+include/specsim3d.h) and the Fortran driver bin/specsim3d that keeps the reference program's
+command line, parameter file, stdout and output files.  This module is the Python binding of
+the same C ABI (ctypes), used by tests, bench.py and Python callers.  It has no fallback:
+every call runs the HIP library, and importing a missing library raises.
+
+Reference: src/specsim3d.f90 of stebond/specsim3d (the hot block f90:1234-1345).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libspecsim3d.so")
+EXE_PATH = os.path.join(PKG, "bin", "specsim3d")
+
+FP64 = 0
+FP32 = 1
+
+ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device"}
+
+
+class SS3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"specsim3d error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Structure(ctypes.Structure):
+    """ss3d_structure: the scalar members of the reference's `type structure` (f90:95-100)."""
+    _fields_ = [("itype", ctypes.c_int32), ("pad_", ctypes.c_int32), ("sill", ctypes.c_double),
+                ("ahmax", ctypes.c_double), ("ahmin", ctypes.c_double), ("avert", ctypes.c_double),
+                ("az", ctypes.c_double), ("dip", ctypes.c_double), ("plg", ctypes.c_double)]
+
+
+class SpecDiag(ctypes.Structure):
+    _fields_ = [("max_imag", ctypes.c_double), ("max_real", ctypes.c_double), ("c0", ctypes.c_double),
+                ("spos", ctypes.c_double), ("sneg", ctypes.c_double)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# exported symbols and their signatures (must match include/specsim3d.h)
+SIGNATURES = {
+    "ss3d_version": (ctypes.c_int, []),
+    "ss3d_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64]),
+    "ss3d_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ss3d_context_destroy": (ctypes.c_int, [_vp]),
+    "ss3d_context_synchronize": (ctypes.c_int, [_vp]),
+    "ss3d_periodic_grid": (ctypes.c_int, [ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _ip, _ip, _ip]),
+    "ss3d_spectrum_create": (ctypes.c_int, [_vp, ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
+                                            ctypes.POINTER(SpecDiag)]),
+    "ss3d_spectrum_destroy": (ctypes.c_int, [_vp]),
+    "ss3d_spectrum_bytes": (ctypes.c_int64, [_vp]),
+    "ss3d_spectrum_amplitude": (ctypes.c_int, [_vp, _dp]),
+    "ss3d_draws_per_realization": (ctypes.c_int64, [ctypes.c_int] * 4),
+    "ss3d_realization_state": (ctypes.c_int64, [ctypes.c_int] * 5 + [ctypes.c_int64]),
+    "ss3d_realize": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _dp, _dp, _dp]),
+    "ss3d_realize_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+}
+
+
+def lib():
+    """Load libspecsim3d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libspecsim3d.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().ss3d_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    if rc != 0:
+        raise SS3DError(rc, last_error())
+
+
+def structures(sts):
+    """sts: iterable of (itype, sill, ahmax, ahmin, avert, azimuth, dip, plunge)."""
+    sts = list(sts)
+    arr = (Structure * max(len(sts), 1))()
+    for i, s in enumerate(sts):
+        arr[i].itype = int(s[0])
+        (arr[i].sill, arr[i].ahmax, arr[i].ahmin, arr[i].avert, arr[i].az, arr[i].dip,
+         arr[i].plg) = map(float, s[1:8])
+    return arr, len(sts)
+
+
+def periodic_grid(sts, dx, dy, dz, nx, ny, nz, ipad):
+    arr, n = structures(sts)
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().ss3d_periodic_grid(arr, n, dx, dy, dz, nx, ny, nz, ipad, ctypes.byref(a), ctypes.byref(b),
+                                   ctypes.byref(c)))
+    return a.value, b.value, c.value
+
+
+def draws_per_realization(npx, npy, npz, igauss):
+    return lib().ss3d_draws_per_realization(npx, npy, npz, igauss)
+
+
+def realization_state(iseed, npx, npy, npz, igauss, r):
+    return lib().ss3d_realization_state(iseed, npx, npy, npz, igauss, r)
+
+
+class Spectrum:
+    """Device-resident spectral density of one (model, periodic grid) (f90:1256-1297)."""
+
+    def __init__(self, engine, handle, grid, diag):
+        self.engine = engine
+        self.handle = handle
+        self.grid = grid
+        self.diag = diag
+
+    def amplitude(self):
+        npx, npy, npz = self.grid
+        out = np.empty(npx * npy * npz)
+        check(lib().ss3d_spectrum_amplitude(self.handle, out.ctypes.data_as(_dp)))
+        return out.reshape(npz, npy, npx)
+
+    def nbytes(self):
+        return lib().ss3d_spectrum_bytes(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().ss3d_spectrum_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One HIP context (device, stream, twiddle tables, work buffers)."""
+
+    def __init__(self, device=0):
+        self.handle = _vp()
+        check(lib().ss3d_context_create(device, ctypes.byref(self.handle)))
+
+    def spectrum(self, sts, nugget, dx, dy, dz, npx, npy, npz):
+        arr, n = structures(sts)
+        h = _vp()
+        d = SpecDiag()
+        check(lib().ss3d_spectrum_create(self.handle, arr, n, nugget, dx, dy, dz, npx, npy, npz, ctypes.byref(h),
+                                         ctypes.byref(d)))
+        diag = dict(max_imag=d.max_imag, max_real=d.max_real, c0=d.c0, spos=d.spos, sneg=d.sneg)
+        return Spectrum(self, h, (npx, npy, npz), diag)
+
+    def realize(self, spec, iseed, igauss, r0, nreal, nx, ny, nz, precision=FP64):
+        """Realizations r0..r0+nreal-1 -> (field[nreal, nz, ny, nx], mean, var, maximag)."""
+        dt = np.float64 if precision == FP64 else np.float32
+        out = np.empty((nreal, nz, ny, nx), dtype=dt)
+        mean, var, mi = np.empty(nreal), np.empty(nreal), np.empty(nreal)
+        check(lib().ss3d_realize(self.handle, spec.handle, iseed, igauss, r0, nreal, nx, ny, nz, precision,
+                                 out.ctypes.data_as(_vp), mean.ctypes.data_as(_dp), var.ctypes.data_as(_dp),
+                                 mi.ctypes.data_as(_dp)))
+        return out, mean, var, mi
+
+    def realize_device(self, spec, iseed, igauss, r0, nreal, nx, ny, nz, precision, dev_out, dev_stats=None):
+        """Asynchronous variant: dev_out / dev_stats are device addresses (ints)."""
+        check(lib().ss3d_realize_device(self.handle, spec.handle, iseed, igauss, r0, nreal, nx, ny, nz, precision,
+                                        _vp(dev_out), _vp(dev_stats) if dev_stats else None))
+
+    def synchronize(self):
+        check(lib().ss3d_context_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().ss3d_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def simulate(engine, sts, nugget, dx, dy, dz, nx, ny, nz, ipad, iseed, nreal, igauss, precision=FP64):
+    """Whole unconditional run as the reference's main block (f90:1234-1345)."""
+    grid = periodic_grid(sts, dx, dy, dz, nx, ny, nz, ipad)
+    spec = engine.spectrum(sts, nugget, dx, dy, dz, *grid)
+    try:
+        res, mean, var, mi = engine.realize(spec, iseed, igauss, 0, nreal, nx, ny, nz, precision)
+    finally:
+        diag = spec.diag
+        spec.close()
+    return dict(res=res, mean=mean, var=var, maximag=mi, diag=diag, np=grid)

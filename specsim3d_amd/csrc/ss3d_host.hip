@@ -1,0 +1,795 @@
+// ss3d_host.cpp -- host side of libspecsim3d.so: the C ABI (include/specsim3d.h), grid sizing,
+// twiddle tables, spectrum handles, random-stream tables and the pass plans.
+//
+// Reference counterparts (src/specsim3d.f90): setrot 107-139, cutoff 142-150, nextpow2 181-188,
+// sizing 1234-1254, covariance grid + spectrum 1256-1297, seeding 1302-1303, realization loop
+// 1305-1344.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/specsim3d.h"
+#include "ss3d_internal.h"
+
+using namespace ss3d;
+
+// ---------------------------------------------------------------- errors
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? SS3D_ENOMEM : SS3D_EHIP, "%s: %s (%s:%d)", #expr, \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                                \
+    } while (0)
+
+#define SS3D_TRY(expr)             \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != SS3D_OK) return r_; \
+    } while (0)
+
+constexpr double kPi = 3.14159265358979323846;  // f90:91
+
+// ---------------------------------------------------------------- host model arithmetic
+// GSLIB scaled rotation, f90:107-139 (row-major rot, axes).
+void setrot(const ss3d_structure& s, double rot[9], double axes[9]) {
+    const double deg = kPi / 180.0;
+    const double alpha = (s.az >= 0.0 && s.az < 270.0) ? (90.0 - s.az) * deg : (450.0 - s.az) * deg;
+    const double beta = -s.dip * deg, theta = s.plg * deg;
+    const double sina = std::sin(alpha), cosa = std::cos(alpha);
+    const double sinb = std::sin(beta), cosb = std::cos(beta);
+    const double sint = std::sin(theta), cost = std::cos(theta);
+    const double anis1 = s.ahmin / s.ahmax, anis2 = s.avert / s.ahmax;
+    const double afac1 = 1.0 / std::max(anis1, 1.0e-20), afac2 = 1.0 / std::max(anis2, 1.0e-20);
+    rot[0] = cosb * cosa;
+    rot[1] = cosb * sina;
+    rot[2] = -sinb;
+    rot[3] = afac1 * (-(cost * sina) + sint * sinb * cosa);
+    rot[4] = afac1 * (cost * cosa + sint * sinb * sina);
+    rot[5] = afac1 * (sint * cosb);
+    rot[6] = afac2 * (sint * sina + cost * sinb * cosa);
+    rot[7] = afac2 * (-(sint * cosa) + cost * sinb * sina);
+    rot[8] = afac2 * (cost * cosb);
+    for (int c = 0; c < 3; ++c) {
+        axes[c] = rot[c] * s.ahmax;
+        axes[3 + c] = rot[3 + c] / afac1 * s.ahmin;
+        axes[6 + c] = rot[6 + c] / afac2 * s.avert;
+    }
+}
+
+double cutoff(int itype) { return itype == 1 ? 1.0 : (itype == 2 ? 2.3 : 1.6); }  // f90:142-150
+
+int64_t nextpow2(int64_t n) {  // f90:181-188
+    int64_t m = 1;
+    while (m < n) m *= 2;
+    return m;
+}
+
+int ilog2(int64_t n) {
+    int l = 0;
+    while ((int64_t(1) << l) < n) ++l;
+    return l;
+}
+
+int check_structures(const ss3d_structure* st, int nst) {
+    if (!st || nst < 1 || nst > kMaxStruct) return fail(SS3D_EINVAL, "nstruct out of range (1..%d): %d", kMaxStruct, nst);
+    for (int s = 0; s < nst; ++s) {
+        if (st[s].itype < 1 || st[s].itype > 3)
+            return fail(SS3D_EINVAL, "structure %d: itype must be 1, 2 or 3 (got %d)", s + 1, st[s].itype);
+        if (!(st[s].ahmax > 0) || !(st[s].ahmin > 0) || !(st[s].avert > 0))
+            return fail(SS3D_EINVAL, "ranges must be positive");
+    }
+    return SS3D_OK;
+}
+
+// ---------------------------------------------------------------- Park-Miller on the host
+uint32_t pm_mul(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) % kPmM); }
+uint32_t pm_pow(uint32_t a, uint64_t e) {
+    e %= (kPmM - 1);  // a^(M-1) = 1, M prime
+    uint32_t r = 1, x = a % kPmM;
+    while (e) {
+        if (e & 1) r = pm_mul(r, x);
+        x = pm_mul(x, x);
+        e >>= 1;
+    }
+    return r;
+}
+// f90:1302-1303: state = mod(|iseed|, 2147483646) + 1, then 10 draws discarded.
+uint32_t seed_state(int iseed) {
+    int64_t a = iseed < 0 ? -(int64_t)iseed : (int64_t)iseed;
+    uint32_t s = (uint32_t)(a % 2147483646LL + 1);
+    return pm_mul(s, pm_pow(kPmA, 10));
+}
+
+int64_t draws_per_realization(int npx, int npy, int npz, int igauss) {
+    int64_t n = (int64_t)npx * npy * npz;
+    int64_t nsc = (npx >= 2 ? 2 : 1) * (npy >= 2 ? 2 : 1) * (npz >= 2 ? 2 : 1);
+    return (igauss ? 2 : 1) * ((n + nsc) / 2);
+}
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int ensure(size_t bytes, hipStream_t s) {
+        if (bytes <= cap) return SS3D_OK;
+        if (p) {
+            HIP_TRY(hipStreamSynchronize(s));
+            HIP_TRY(hipFree(p));
+            p = nullptr;
+            cap = 0;
+        }
+        HIP_TRY(hipMalloc(&p, bytes));
+        cap = bytes;
+        return SS3D_OK;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+__global__ void rbase_kernel(uint32_t s10, uint64_t dmod, uint64_t r0, int n, uint32_t* out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t m1 = kPmM - 1;
+    uint64_t r = (r0 + (uint64_t)t) % m1;
+    uint64_t e = (r * dmod + 1) % m1;
+    uint64_t x = kPmA, acc = 1;
+    while (e) {
+        if (e & 1) acc = (acc * x) % kPmM;
+        x = (x * x) % kPmM;
+        e >>= 1;
+    }
+    out[t] = (uint32_t)((acc * s10) % kPmM);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- context and spectrum
+struct ss3d_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // twiddles for lengths 2^1..2^kMaxLogN: entry k of length n at offset (n/2 - 1) + k
+    DevBuf tw64f, tw64i, tw32i;
+    DevBuf wa, wb, partial, stats, dout, rbase, diag;
+    int target_e64 = 4096;   // elements per FP64 tile
+    int target_e32 = 8192;   // elements per FP32 tile
+};
+
+struct RngTables {
+    bool ready = false;
+    DevBuf rowf, txp;
+};
+
+struct ss3d_spectrum {
+    ss3d_context* ctx = nullptr;
+    int npx = 0, npy = 0, npz = 0, hx = 0;
+    DevBuf amp64, amp32h;
+    RngTables rng[2];
+    ss3d_spec_diag diag{};
+};
+
+namespace {
+
+const void* tw_ptr(const DevBuf& b, int logn, size_t esz) {
+    size_t off = logn >= 1 ? ((size_t(1) << (logn - 1)) - 1) : 0;
+    return static_cast<const char*>(b.p) + off * esz;
+}
+
+int build_twiddles(ss3d_context* c) {
+    const size_t total = (size_t(1) << kMaxLogN) - 1;  // sum of n/2 over n = 2..2^kMaxLogN
+    std::vector<double> f(2 * total), iv(2 * total);
+    std::vector<float> i32(2 * total);
+    for (int L = 1; L <= kMaxLogN; ++L) {
+        const int n = 1 << L;
+        const size_t off = (size_t(1) << (L - 1)) - 1;
+        for (int k = 0; k < n / 2; ++k) {
+            // f90:249: cmplx(cos(2.0d0*pi*k/n), isign*sin(2.0d0*pi*k/n))
+            const double ang = 2.0 * kPi * (double)k / (double)n;
+            const double cs = std::cos(ang), sn = std::sin(ang);
+            f[2 * (off + k)] = cs;
+            f[2 * (off + k) + 1] = -1.0 * sn;
+            iv[2 * (off + k)] = cs;
+            iv[2 * (off + k) + 1] = 1.0 * sn;
+            i32[2 * (off + k)] = (float)cs;
+            i32[2 * (off + k) + 1] = (float)sn;
+        }
+    }
+    SS3D_TRY(c->tw64f.ensure(f.size() * 8, c->stream));
+    SS3D_TRY(c->tw64i.ensure(iv.size() * 8, c->stream));
+    SS3D_TRY(c->tw32i.ensure(i32.size() * 4, c->stream));
+    HIP_TRY(hipMemcpy(c->tw64f.p, f.data(), f.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->tw64i.p, iv.data(), iv.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->tw32i.p, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
+    return SS3D_OK;
+}
+
+// Random-stream tables of a grid (closed form of the fill walk f90:1305-1334):
+//   txp[i]  = a^(g*i)
+//   rowf[r] = a^(g*q0(r)), q0 = index among drawing points of the first point of drawing row r.
+// Drawing planes: k = 0 (self-conjugate), 0 < k < npz/2 (forward), k = npz/2 (self-conjugate).
+// In a self-conjugate plane, rows j = 0 and j = npy/2 draw for i <= npx/2, rows 0 < j < npy/2
+// for every i; in a forward plane every row draws for every i.
+int build_rng(ss3d_spectrum* sp, int igauss) {
+    RngTables& t = sp->rng[igauss];
+    if (t.ready) return SS3D_OK;
+    const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
+    const uint64_t g = igauss ? 2 : 1;
+    std::vector<uint32_t> tx(npx), rowf((size_t)npy * npz, 0u);
+    const uint32_t ax = pm_pow(kPmA, g);
+    tx[0] = 1;
+    for (int i = 1; i < npx; ++i) tx[i] = pm_mul(tx[i - 1], ax);
+    const int64_t hxc = npx >= 2 ? npx / 2 + 1 : 1;
+    const int64_t hp = npy >= 2 ? (int64_t)(npy / 2 - 1) * npx + 2 * hxc : hxc;
+    const int64_t fp = (int64_t)npy * npx;
+    const uint32_t arow = pm_pow(kPmA, g * (uint64_t)npx);
+    for (int k = 0; k <= npz / 2 && k < npz; ++k) {
+        const int64_t ps = (k == 0) ? 0 : hp + (int64_t)(k - 1) * fp;
+        const bool self_plane = (k == 0) || (npz >= 2 && k == npz / 2);
+        uint32_t* rk = rowf.data() + (size_t)npy * k;
+        if (!self_plane) {
+            uint32_t v = pm_pow(kPmA, g * (uint64_t)ps);
+            for (int j = 0; j < npy; ++j) {
+                rk[j] = v;
+                v = pm_mul(v, arow);
+            }
+        } else {
+            rk[0] = pm_pow(kPmA, g * (uint64_t)ps);
+            if (npy >= 2) {
+                uint32_t v = pm_pow(kPmA, g * (uint64_t)(ps + hxc));
+                for (int j = 1; j <= npy / 2; ++j) {
+                    rk[j] = v;
+                    v = pm_mul(v, arow);
+                }
+            }
+        }
+    }
+    hipStream_t s = sp->ctx->stream;
+    SS3D_TRY(t.txp.ensure(tx.size() * 4, s));
+    SS3D_TRY(t.rowf.ensure(rowf.size() * 4, s));
+    HIP_TRY(hipMemcpy(t.txp.p, tx.data(), tx.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.rowf.p, rowf.data(), rowf.size() * 4, hipMemcpyHostToDevice));
+    t.ready = true;
+    return SS3D_OK;
+}
+
+// Tile shape of a pass: returns threads per block and sets logp.
+struct Shape {
+    int nt, logp;
+};
+Shape contig_shape(int logn, int64_t nrows, int target_e, int emax) {
+    int64_t e = std::max<int64_t>(target_e, int64_t(1) << logn);
+    // do not make tiles much larger than the work
+    int64_t work = (int64_t)nextpow2(std::max<int64_t>(nrows, 1)) << logn;
+    e = std::min(e, std::max<int64_t>(work, int64_t(1) << logn));
+    e = std::max<int64_t>(e, 64 * kEPT);
+    e = std::min<int64_t>(e, emax);
+    return {(int)(e / kEPT), ilog2(e) - logn};
+}
+Shape strided_shape(int logn, int64_t n0, int row_bytes_target, int esz, int target_e, int emax) {
+    int64_t tx = std::max<int64_t>(1, row_bytes_target / esz);
+    tx = std::min<int64_t>(tx, nextpow2(std::max<int64_t>(n0, 1)));
+    int64_t e = tx << logn;
+    if (e < target_e) e = std::min<int64_t>(target_e, nextpow2(std::max<int64_t>(n0, 1)) << logn);
+    e = std::max<int64_t>(e, 64 * kEPT);
+    e = std::min<int64_t>(e, emax);
+    return {(int)(e / kEPT), ilog2(e) - logn};
+}
+
+struct Pass {
+    int prec;  // 64 or 32
+    int in, out;
+    Shape sh;
+    int nblocks;
+    PassArgs a;
+};
+
+int run_pass(ss3d_context* c, Pass& p, int nbatch) {
+    p.a.logp = p.sh.logp;
+    const int P = 1 << p.sh.logp;
+    if (p.a.axis == 0) {
+        p.nblocks = (int)((p.a.nrows + P - 1) / P);
+    } else {
+        p.a.ntiles0 = (p.a.n0 + P - 1) / P;
+        p.nblocks = p.a.ntiles0 * p.a.nthird;
+    }
+    p.a.nblocks = p.nblocks;
+    if (p.nblocks <= 0) return fail(SS3D_EINVAL, "empty pass");
+    if (p.out == OUT_SPEC || p.out == OUT_REAL) {
+        SS3D_TRY(c->partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), c->stream));
+        p.a.partial = c->partial.as<double>();
+    }
+    hipError_t e = p.prec == 64 ? launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream)
+                                : launch_pass32(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream);
+    if (e != hipSuccess) return fail(SS3D_EHIP, "pass launch (prec %d in %d out %d nt %d): %s", p.prec, p.in, p.out,
+                                     p.sh.nt, hipGetErrorString(e));
+    return SS3D_OK;
+}
+
+PassArgs base_args(const ss3d_spectrum* sp) {
+    PassArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.npx = sp->npx;
+    a.npy = sp->npy;
+    a.npz = sp->npz;
+    a.hx = sp->hx;
+    return a;
+}
+
+int check_grid(int npx, int npy, int npz) {
+    for (int v : {npx, npy, npz})
+        if (v < 1 || (v & (v - 1)) != 0) return fail(SS3D_EINVAL, "periodic grid sizes must be powers of two (got %d)", v);
+    return SS3D_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int ss3d_version(void) { return 10000; }
+
+int ss3d_last_error(char* buf, int64_t len) {
+    if (!buf || len <= 0) return SS3D_EINVAL;
+    std::snprintf(buf, (size_t)len, "%s", g_last_error.c_str());
+    return SS3D_OK;
+}
+
+int ss3d_context_create(int device, ss3d_context** out) {
+    if (!out) return fail(SS3D_EINVAL, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(SS3D_ENODEV, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(SS3D_EINVAL, "device %d out of range (0..%d)", device, ndev - 1);
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<ss3d_context> c(new ss3d_context);
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    SS3D_TRY(build_twiddles(c.get()));
+    *out = c.release();
+    return SS3D_OK;
+}
+
+int ss3d_context_destroy(ss3d_context* ctx) {
+    if (!ctx) return SS3D_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    (void)hipStreamDestroy(s);
+    return SS3D_OK;
+}
+
+int ss3d_context_synchronize(ss3d_context* ctx) {
+    if (!ctx) return fail(SS3D_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SS3D_OK;
+}
+
+int ss3d_periodic_grid(const ss3d_structure* st, int nst, double dx, double dy, double dz, int nx, int ny, int nz,
+                       int ipad, int* npx, int* npy, int* npz) {
+    SS3D_TRY(check_structures(st, nst));
+    if (nx < 1 || ny < 1 || nz < 1) return fail(SS3D_EINVAL, "grid sizes must be positive");
+    if (!npx || !npy || !npz) return fail(SS3D_EINVAL, "null output pointer");
+    int64_t px = nextpow2(nx), py = nextpow2(ny), pz = nextpow2(nz);
+    if (ipad == 1) {
+        // reach of the covariance along each axis in cells (f90:1236-1245)
+        double ex = 0, ey = 0, ez = 0;
+        for (int s = 0; s < nst; ++s) {
+            double rot[9], axes[9];
+            setrot(st[s], rot, axes);
+            const double co = cutoff(st[s].itype);
+            ex = std::max(ex, co * std::sqrt(axes[0] * axes[0] + axes[3] * axes[3] + axes[6] * axes[6]) / dx);
+            ey = std::max(ey, co * std::sqrt(axes[1] * axes[1] + axes[4] * axes[4] + axes[7] * axes[7]) / dy);
+            ez = std::max(ez, co * std::sqrt(axes[2] * axes[2] + axes[5] * axes[5] + axes[8] * axes[8]) / dz);
+        }
+        const int64_t rx = (int64_t)std::ceil(ex), ry = (int64_t)std::ceil(ey), rz = (int64_t)std::ceil(ez);
+        px = nextpow2(std::max<int64_t>(nx + rx, 2 * rx));
+        py = nextpow2(std::max<int64_t>(ny + ry, 2 * ry));
+        pz = nz > 1 ? nextpow2(std::max<int64_t>(nz + rz, 2 * rz)) : 1;
+    } else if (ipad != 0) {
+        return fail(SS3D_EINVAL, "ipad must be 0 or 1");
+    }
+    if (px > (1 << 30) || py > (1 << 30) || pz > (1 << 30)) return fail(SS3D_ETOOBIG, "periodic grid axis too long");
+    *npx = (int)px;
+    *npy = (int)py;
+    *npz = (int)pz;
+    return SS3D_OK;
+}
+
+int64_t ss3d_draws_per_realization(int npx, int npy, int npz, int igauss) {
+    return draws_per_realization(npx, npy, npz, igauss);
+}
+
+int64_t ss3d_realization_state(int iseed, int npx, int npy, int npz, int igauss, int64_t r) {
+    const uint64_t d = (uint64_t)draws_per_realization(npx, npy, npz, igauss) % (kPmM - 1);
+    const uint64_t e = (((uint64_t)r % (kPmM - 1)) * d) % (kPmM - 1);
+    return pm_mul(seed_state(iseed), pm_pow(kPmA, e));
+}
+
+int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, double nugget, double dx, double dy,
+                         double dz, int npx, int npy, int npz, ss3d_spectrum** out, ss3d_spec_diag* diag) {
+    if (!c || !out) return fail(SS3D_EINVAL, "null argument");
+    *out = nullptr;
+    SS3D_TRY(check_structures(st, nst));
+    SS3D_TRY(check_grid(npx, npy, npz));
+    const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
+    if (std::max({lx, ly, lz}) > 13)
+        return fail(SS3D_ETOOBIG, "periodic axis longer than 8192 is not supported by the FP64 spectrum passes");
+    HIP_TRY(hipSetDevice(c->device));
+    std::unique_ptr<ss3d_spectrum> sp(new ss3d_spectrum);
+    sp->ctx = c;
+    sp->npx = npx;
+    sp->npy = npy;
+    sp->npz = npz;
+    sp->hx = npx >= 2 ? npx / 2 + 1 : 1;
+    const int64_t ntot = (int64_t)npx * npy * npz;
+    const int64_t nh = (int64_t)sp->hx * npy * npz;
+    SS3D_TRY(sp->amp64.ensure(ntot * sizeof(double), c->stream));
+    SS3D_TRY(sp->amp32h.ensure(nh * sizeof(float), c->stream));
+    SS3D_TRY(c->wa.ensure(ntot * 2 * sizeof(double), c->stream));
+    SS3D_TRY(c->diag.ensure(8 * sizeof(double), c->stream));
+
+    PassArgs a = base_args(sp.get());
+    a.nst = nst;
+    a.nugget = nugget;
+    a.dx = dx;
+    a.dy = dy;
+    a.dz = dz;
+    double c0 = nugget;  // C(0) = covariance at lag 0 (f90:1270)
+    for (int s = 0; s < nst; ++s) {
+        double rot[9], axes[9];
+        setrot(st[s], rot, axes);
+        a.st[s].itype = st[s].itype;
+        a.st[s].sill = st[s].sill;
+        a.st[s].ahmax = st[s].ahmax;
+        std::memcpy(a.st[s].rot, rot, sizeof rot);
+        c0 = c0 + st[s].sill * 1.0;
+    }
+    a.amp64_out = sp->amp64.as<double>();
+    a.amp32h_out = sp->amp32h.as<float>();
+    const int emax = 8192;
+    const bool has_y = npy > 1, has_z = npz > 1;
+    // x pass: covariance generated in registers, transformed along x (isign = -1)
+    Pass px{64, IN_COV, (has_y || has_z) ? OUT_STORE : OUT_SPEC, contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a};
+    px.a.axis = 0;
+    px.a.logn = lx;
+    px.a.nrows = npy * npz;
+    px.a.n_out = npx;
+    px.a.out = c->wa.p;
+    px.a.out_row = npx;
+    px.a.tw = tw_ptr(c->tw64f, lx, 16);
+    SS3D_TRY(run_pass(c, px, 1));
+    Pass* last = &px;
+    Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a};
+    if (has_y) {
+        py.a.axis = 1;
+        py.a.logn = ly;
+        py.a.n0 = npx;
+        py.a.nthird = npz;
+        py.a.n_out = npy;
+        py.a.in = c->wa.p;
+        py.a.out = c->wa.p;
+        py.a.in_sA = py.a.out_sA = npx;
+        py.a.in_sT = py.a.out_sT = (int64_t)npx * npy;
+        py.a.tw = tw_ptr(c->tw64f, ly, 16);
+        SS3D_TRY(run_pass(c, py, 1));
+        last = &py;
+    }
+    Pass pz{64, IN_LOAD, OUT_SPEC, strided_shape(lz, npx, 128, 16, c->target_e64, emax), 0, a};
+    if (has_z) {
+        pz.a.axis = 2;
+        pz.a.logn = lz;
+        pz.a.n0 = npx;
+        pz.a.nthird = npy;
+        pz.a.n_out = npz;
+        pz.a.in = c->wa.p;
+        pz.a.out = c->wa.p;
+        pz.a.in_sA = pz.a.out_sA = (int64_t)npx * npy;
+        pz.a.in_sT = pz.a.out_sT = npx;
+        pz.a.tw = tw_ptr(c->tw64f, lz, 16);
+        SS3D_TRY(run_pass(c, pz, 1));
+        last = &pz;
+    }
+    hipError_t e = launch_diag_finalize(c->partial.as<double>(), last->nblocks, c->diag.as<double>(), c->stream);
+    if (e != hipSuccess) return fail(SS3D_EHIP, "diag finalize: %s", hipGetErrorString(e));
+    double d4[4];
+    HIP_TRY(hipMemcpyAsync(d4, c->diag.p, sizeof d4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    sp->diag.max_imag = d4[0];
+    sp->diag.max_real = d4[1];
+    sp->diag.sneg = d4[2];
+    sp->diag.spos = d4[3];
+    sp->diag.c0 = c0;
+    if (diag) *diag = sp->diag;
+    *out = sp.release();
+    return SS3D_OK;
+}
+
+int ss3d_spectrum_destroy(ss3d_spectrum* sp) {
+    if (!sp) return SS3D_OK;
+    (void)hipSetDevice(sp->ctx->device);
+    (void)hipStreamSynchronize(sp->ctx->stream);
+    delete sp;
+    return SS3D_OK;
+}
+
+int64_t ss3d_spectrum_bytes(const ss3d_spectrum* sp) {
+    if (!sp) return 0;
+    return (int64_t)(sp->amp64.cap + sp->amp32h.cap + sp->rng[0].rowf.cap + sp->rng[0].txp.cap + sp->rng[1].rowf.cap +
+                     sp->rng[1].txp.cap);
+}
+
+int ss3d_spectrum_amplitude(const ss3d_spectrum* sp, double* host_amp) {
+    if (!sp || !host_amp) return fail(SS3D_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(sp->ctx->device));
+    HIP_TRY(hipMemcpyAsync(host_amp, sp->amp64.p, (size_t)sp->npx * sp->npy * sp->npz * sizeof(double),
+                           hipMemcpyDeviceToHost, sp->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(sp->ctx->stream));
+    return SS3D_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- realizations
+namespace {
+
+// Enqueue realizations r0..r0+nb-1 into dout (device) and their statistics into dstats.
+int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nb, int nx, int ny,
+                  int nz, int prec, void* dout, double* dstats) {
+    const int npx = sp->npx, npy = sp->npy, npz = sp->npz, hx = sp->hx;
+    const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
+    SS3D_TRY(build_rng(sp, igauss));
+    SS3D_TRY(c->rbase.ensure((size_t)nb * 4, c->stream));
+    {
+        const uint64_t dmod = (uint64_t)draws_per_realization(npx, npy, npz, igauss) % (kPmM - 1);
+        hipLaunchKernelGGL(rbase_kernel, dim3((nb + 63) / 64), dim3(64), 0, c->stream, seed_state(iseed), dmod,
+                           (uint64_t)r0, nb, c->rbase.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+    }
+    PassArgs a = base_args(sp);
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.igauss = igauss;
+    a.amp64 = sp->amp64.as<double>();
+    a.amp32h = sp->amp32h.as<float>();
+    a.rowf = sp->rng[igauss].rowf.as<uint32_t>();
+    a.txp = sp->rng[igauss].txp.as<uint32_t>();
+    a.rbase = c->rbase.as<uint32_t>();
+    const bool has_y = npy > 1, has_z = npz > 1;
+    const int64_t ncrop = (int64_t)nx * ny * nz;
+    Pass* last = nullptr;
+    Pass p1, p2, p3;
+    if (prec == SS3D_FP64) {
+        // x -> y -> z, C2C, pruned to the crop after each pass (f90:1335-1342)
+        const int emax = 8192;
+        const int64_t b1 = (int64_t)nx * npy * npz, b2 = (int64_t)nx * ny * npz;
+        SS3D_TRY(c->wa.ensure((size_t)b1 * nb * 16, c->stream));
+        SS3D_TRY(c->wb.ensure((size_t)std::max<int64_t>(b2, 1) * nb * 16, c->stream));
+        p1 = Pass{64, IN_FILL64, (has_y || has_z) ? OUT_STORE : OUT_REAL,
+                  contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a};
+        p1.a.axis = 0;
+        p1.a.logn = lx;
+        p1.a.nrows = npy * npz;
+        p1.a.n_out = nx;
+        p1.a.tw = tw_ptr(c->tw64i, lx, 16);
+        if (has_y || has_z) {
+            p1.a.out = c->wa.p;
+            p1.a.out_row = nx;
+            p1.a.out_bstride = b1;
+        } else {
+            p1.a.out = dout;
+            p1.a.out_row = nx;
+            p1.a.out_bstride = ncrop;
+        }
+        SS3D_TRY(run_pass(c, p1, nb));
+        last = &p1;
+        if (has_y) {
+            p2 = Pass{64, IN_LOAD, has_z ? OUT_STORE : OUT_REAL, strided_shape(ly, nx, 128, 16, c->target_e64, emax), 0, a};
+            p2.a.axis = 1;
+            p2.a.logn = ly;
+            p2.a.n0 = nx;
+            p2.a.nthird = npz;
+            p2.a.n_out = ny;
+            p2.a.in = c->wa.p;
+            p2.a.in_sA = nx;
+            p2.a.in_sT = (int64_t)nx * npy;
+            p2.a.in_bstride = b1;
+            p2.a.out = has_z ? c->wb.p : dout;
+            p2.a.out_sA = nx;
+            p2.a.out_sT = (int64_t)nx * ny;
+            p2.a.out_bstride = has_z ? b2 : ncrop;
+            p2.a.tw = tw_ptr(c->tw64i, ly, 16);
+            SS3D_TRY(run_pass(c, p2, nb));
+            last = &p2;
+        }
+        if (has_z) {
+            p3 = Pass{64, IN_LOAD, OUT_REAL, strided_shape(lz, nx, 128, 16, c->target_e64, emax), 0, a};
+            p3.a.axis = 2;
+            p3.a.logn = lz;
+            p3.a.n0 = nx;
+            p3.a.nthird = ny;
+            p3.a.n_out = nz;
+            p3.a.in = has_y ? c->wb.p : c->wa.p;
+            p3.a.in_sA = (int64_t)nx * ny;
+            p3.a.in_sT = nx;
+            p3.a.in_bstride = has_y ? b2 : b1;
+            p3.a.out = dout;
+            p3.a.out_sA = (int64_t)nx * ny;
+            p3.a.out_sT = nx;
+            p3.a.out_bstride = ncrop;
+            p3.a.tw = tw_ptr(c->tw64i, lz, 16);
+            SS3D_TRY(run_pass(c, p3, nb));
+            last = &p3;
+        }
+    } else {
+        // FP32: z -> y over the half spectrum kx <= npx/2, then the C2R x pass
+        const int emax = 16384;
+        if (lx > 14 || ly > 14 || lz > 14) return fail(SS3D_ETOOBIG, "periodic axis longer than 16384");
+        const int64_t bz = (int64_t)hx * npy * nz, by = (int64_t)hx * ny * nz;
+        SS3D_TRY(c->wa.ensure((size_t)std::max<int64_t>(bz, 1) * nb * 8, c->stream));
+        SS3D_TRY(c->wb.ensure((size_t)std::max<int64_t>(by, 1) * nb * 8, c->stream));
+        const void* prev = nullptr;
+        int64_t prev_b = 0;
+        if (has_z) {
+            p1 = Pass{32, IN_FILL32H, OUT_STORE, strided_shape(lz, hx, 128, 8, c->target_e32, emax), 0, a};
+            p1.a.axis = 2;
+            p1.a.logn = lz;
+            p1.a.n0 = hx;
+            p1.a.nthird = npy;
+            p1.a.n_out = nz;
+            p1.a.out = c->wa.p;
+            p1.a.out_sA = (int64_t)hx * npy;
+            p1.a.out_sT = hx;
+            p1.a.out_bstride = bz;
+            p1.a.tw = tw_ptr(c->tw32i, lz, 8);
+            SS3D_TRY(run_pass(c, p1, nb));
+            prev = c->wa.p;
+            prev_b = bz;
+        }
+        if (has_y) {
+            p2 = Pass{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, strided_shape(ly, hx, 128, 8, c->target_e32, emax), 0, a};
+            p2.a.axis = 1;
+            p2.a.logn = ly;
+            p2.a.n0 = hx;
+            p2.a.nthird = nz;
+            p2.a.n_out = ny;
+            p2.a.in = prev;
+            p2.a.in_sA = hx;
+            p2.a.in_sT = (int64_t)hx * npy;
+            p2.a.in_bstride = prev_b;
+            p2.a.out = c->wb.p;
+            p2.a.out_sA = hx;
+            p2.a.out_sT = (int64_t)hx * ny;
+            p2.a.out_bstride = by;
+            p2.a.tw = tw_ptr(c->tw32i, ly, 8);
+            SS3D_TRY(run_pass(c, p2, nb));
+            prev = c->wb.p;
+            prev_b = by;
+        }
+        p3 = Pass{32, prev ? IN_LOADH : IN_FILLH, OUT_REAL, contig_shape(lx, (int64_t)ny * nz, c->target_e32, emax), 0, a};
+        p3.a.axis = 0;
+        p3.a.logn = lx;
+        p3.a.nrows = ny * nz;
+        p3.a.n_out = nx;
+        p3.a.in = prev;
+        p3.a.in_row = hx;
+        p3.a.in_bstride = prev_b;
+        p3.a.out = dout;
+        p3.a.out_row = nx;
+        p3.a.out_bstride = ncrop;
+        p3.a.tw = tw_ptr(c->tw32i, lx, 8);
+        SS3D_TRY(run_pass(c, p3, nb));
+        last = &p3;
+    }
+    hipError_t e = launch_stats_finalize(c->partial.as<double>(), last->nblocks, nb, (double)ncrop, dstats, c->stream);
+    if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
+    return SS3D_OK;
+}
+
+int check_realize(ss3d_context* c, const ss3d_spectrum* sp, int igauss, int64_t r0, int nreal, int nx, int ny, int nz,
+                  int prec) {
+    if (!c || !sp) return fail(SS3D_EINVAL, "null context or spectrum");
+    if (sp->ctx != c) return fail(SS3D_EINVAL, "spectrum belongs to another context");
+    if (igauss != 0 && igauss != 1) return fail(SS3D_EINVAL, "igauss must be 0 or 1");
+    if (prec != SS3D_FP64 && prec != SS3D_FP32) return fail(SS3D_EINVAL, "precision must be SS3D_FP64 or SS3D_FP32");
+    if (r0 < 0 || nreal < 0) return fail(SS3D_EINVAL, "realization range must be non-negative");
+    if (nx < 1 || ny < 1 || nz < 1 || nx > sp->npx || ny > sp->npy || nz > sp->npz)
+        return fail(SS3D_EINVAL, "output grid %dx%dx%d must be positive and inside the periodic grid %dx%dx%d", nx, ny, nz,
+                    sp->npx, sp->npy, sp->npz);
+    return SS3D_OK;
+}
+
+// realizations per launch: enough to fill the GPU for small grids, bounded by work memory
+int batch_size(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal) {
+    const int64_t per = prec == SS3D_FP64
+                            ? ((int64_t)nx * sp->npy * sp->npz + (int64_t)nx * ny * sp->npz) * 16
+                            : ((int64_t)sp->hx * sp->npy * nz + (int64_t)sp->hx * ny * nz) * 8;
+    const int64_t budget = int64_t(4) << 30;
+    int64_t nb = std::max<int64_t>(1, budget / std::max<int64_t>(per, 1));
+    const int64_t fill = std::max<int64_t>(1, (int64_t(1) << 24) / std::max<int64_t>((int64_t)sp->npx * sp->npy * sp->npz, 1));
+    nb = std::min<int64_t>(nb, std::max<int64_t>(fill, 1));
+    nb = std::min<int64_t>(nb, nreal);
+    return (int)std::max<int64_t>(nb, 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ss3d_realize_device(ss3d_context* c, const ss3d_spectrum* spc, int iseed, int igauss, int64_t r0, int nreal, int nx,
+                        int ny, int nz, int prec, void* dev_out, double* dev_stats) {
+    SS3D_TRY(check_realize(c, spc, igauss, r0, nreal, nx, ny, nz, prec));
+    if (nreal == 0) return SS3D_OK;
+    if (!dev_out) return fail(SS3D_EINVAL, "null device output");
+    HIP_TRY(hipSetDevice(c->device));
+    ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
+    const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
+    const size_t esz = prec == SS3D_FP64 ? 8 : 4;
+    const int64_t ncrop = (int64_t)nx * ny * nz;
+    if (!dev_stats) SS3D_TRY(c->stats.ensure((size_t)nb * 3 * sizeof(double), c->stream));
+    for (int done = 0; done < nreal; done += nb) {
+        const int cnt = std::min(nb, nreal - done);
+        void* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * esz;
+        double* st = dev_stats ? dev_stats + 3 * (size_t)done : c->stats.as<double>();
+        SS3D_TRY(enqueue_batch(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, out, st));
+    }
+    return SS3D_OK;
+}
+
+int ss3d_realize(ss3d_context* c, const ss3d_spectrum* spc, int iseed, int igauss, int64_t r0, int nreal, int nx,
+                 int ny, int nz, int prec, void* host_out, double* mean, double* var, double* maximag) {
+    SS3D_TRY(check_realize(c, spc, igauss, r0, nreal, nx, ny, nz, prec));
+    if (nreal == 0) return SS3D_OK;
+    if (!host_out) return fail(SS3D_EINVAL, "null host output");
+    HIP_TRY(hipSetDevice(c->device));
+    ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
+    const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
+    const size_t esz = prec == SS3D_FP64 ? 8 : 4;
+    const int64_t ncrop = (int64_t)nx * ny * nz;
+    SS3D_TRY(c->dout.ensure((size_t)nb * ncrop * esz, c->stream));
+    SS3D_TRY(c->stats.ensure((size_t)nb * 3 * sizeof(double), c->stream));
+    std::vector<double> hs((size_t)nb * 3);
+    for (int done = 0; done < nreal; done += nb) {
+        const int cnt = std::min(nb, nreal - done);
+        SS3D_TRY(enqueue_batch(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, c->dout.p, c->stats.as<double>()));
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(host_out) + (size_t)done * ncrop * esz, c->dout.p,
+                               (size_t)cnt * ncrop * esz, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(hs.data(), c->stats.p, (size_t)cnt * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < cnt; ++i) {
+            if (mean) mean[done + i] = hs[3 * i];
+            if (var) var[done + i] = hs[3 * i + 1];
+            if (maximag) maximag[done + i] = hs[3 * i + 2];
+        }
+    }
+    return SS3D_OK;
+}
+
+}  // extern "C"

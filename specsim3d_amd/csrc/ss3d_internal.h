@@ -1,0 +1,86 @@
+// ss3d_internal.h -- declarations shared by the host library (ss3d_host.cpp) and the kernel
+// translation units (ss3d_k64.hip, ss3d_k32.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ss3d {
+
+constexpr int kMaxStruct = 10;     // maxst, src/specsim3d.f90:93
+constexpr int kMaxLogN = 14;       // longest axis the LDS-resident passes take: 2^14 (FP32), 2^13 (FP64)
+constexpr uint32_t kPmM = 2147483647u;  // Park-Miller modulus 2^31-1 (f90:193)
+constexpr uint32_t kPmA = 16807u;       // Park-Miller multiplier (f90:193)
+constexpr int kEPT = 16;           // complex elements per thread in every FFT pass
+constexpr int kPadShift = 4;       // LDS padding: one slot after every 16 complex elements
+
+// One covariance structure as the kernels use it (setrot already applied, f90:107-139).
+struct StructDev {
+    int itype;
+    int pad_;
+    double sill, ahmax;
+    double rot[9];  // row-major
+};
+
+// Everything a pass needs.  Passed by value (about 1.3 KB of kernel arguments).
+struct PassArgs {
+    // FFT geometry
+    int logn;   // transform length along the pass axis = 2^logn
+    int logp;   // pencils per tile = 2^logp (tile holds 2^(logn+logp) = NT*kEPT elements)
+    int n_out;  // positions kept along the axis on output (pruning)
+    int axis;   // grid axis of the pass: 0 x (contiguous pencils), 1 y, 2 z (strided tiles)
+    // strided tiles: pencils are consecutive along the contiguous dimension i0
+    int n0;       // active extent of i0
+    int ntiles0;  // ceil(n0 / P)
+    int nthird;   // extent of the remaining dimension
+    // contiguous tiles: whole rows
+    int nrows;
+    int64_t in_sA, in_sT, out_sA, out_sT;  // element strides (axis, third dimension)
+    int64_t in_row, out_row;               // row strides of contiguous tiles
+    int64_t in_bstride, out_bstride;       // per batch member (realization)
+    const void* in;
+    void* out;
+    const void* tw;  // twiddle table of length 2^(logn-1) for this axis and direction
+    // periodic grid and crop
+    int npx, npy, npz, hx;  // hx = npx/2+1 (npx >= 2) or 1: half-spectrum width
+    int nx, ny, nz;
+    // covariance model (spectrum passes)
+    int nst;
+    double nugget, dx, dy, dz;
+    StructDev st[kMaxStruct];
+    // coefficient fill (realization passes)
+    const double* amp64;     // npx*npy*npz
+    const float* amp32h;     // hx*npy*npz
+    const uint32_t* rowf;    // per drawing row: a^(g*(draw-point index of its first point))
+    const uint32_t* txp;     // a^(g*i), i < npx
+    const uint32_t* rbase;   // per realization: state before its first draw, times a
+    int igauss;
+    // outputs besides `out`
+    double* amp64_out;
+    float* amp32h_out;
+    double* partial;  // per block: statistics (3 doubles) or spectrum diagnostics (4 doubles)
+    int nblocks;      // blocks per batch member
+};
+
+enum InMode { IN_COV = 0, IN_FILL64 = 1, IN_LOAD = 2, IN_FILL32H = 3, IN_LOADH = 4, IN_FILLH = 5 };
+enum OutMode { OUT_STORE = 0, OUT_SPEC = 1, OUT_REAL = 2 };
+
+// Kernel launchers (defined in the .hip translation units).  nt: threads per block
+// (64..1024), nbatch: batch members (grid y).  Return hipSuccess or the launch error.
+hipError_t launch_pass64(int in_mode, int out_mode, int nt, int nblocks, int nbatch, const PassArgs& a,
+                         hipStream_t s);
+hipError_t launch_pass32(int in_mode, int out_mode, int nt, int nblocks, int nbatch, const PassArgs& a,
+                         hipStream_t s);
+// Reduce per-block partials: statistics -> (mean, var, max|Im|) per batch member, and
+// spectrum diagnostics -> (max|Im S|, max|Re S|, sneg, spos).
+hipError_t launch_stats_finalize(const double* partial, int nblocks, int nbatch, double ncell, double* stats,
+                                 hipStream_t s);
+hipError_t launch_diag_finalize(const double* partial, int nblocks, double* diag4, hipStream_t s);
+
+// LDS bytes a pass with nt threads uses (dynamic shared memory).
+inline size_t pass_lds_bytes(int nt, size_t elem_bytes) {
+    size_t e = (size_t)nt * kEPT;
+    return (e + (e >> kPadShift)) * elem_bytes;
+}
+
+}  // namespace ss3d

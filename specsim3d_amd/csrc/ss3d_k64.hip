@@ -1,0 +1,74 @@
+// ss3d_k64.hip -- FP64 instantiations of the pass kernel (spectral density and the drop-in
+// realization path) and the reduction finalizers.  Built with -ffp-contract=off: every
+// product and sum is rounded separately, as in the reference build, so butterflies,
+// covariance and coefficients follow the reference's arithmetic.
+#include "ss3d_pass.h"
+
+namespace ss3d {
+
+hipError_t launch_pass64(int in, int out, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    using R = double;
+    if (in == IN_COV && out == OUT_STORE) return launch_nt<R, IN_COV, OUT_STORE>(nt, nblocks, nbatch, a, s);
+    if (in == IN_COV && out == OUT_SPEC) return launch_nt<R, IN_COV, OUT_SPEC>(nt, nblocks, nbatch, a, s);
+    if (in == IN_LOAD && out == OUT_STORE) return launch_nt<R, IN_LOAD, OUT_STORE>(nt, nblocks, nbatch, a, s);
+    if (in == IN_LOAD && out == OUT_SPEC) return launch_nt<R, IN_LOAD, OUT_SPEC>(nt, nblocks, nbatch, a, s);
+    if (in == IN_LOAD && out == OUT_REAL) return launch_nt<R, IN_LOAD, OUT_REAL>(nt, nblocks, nbatch, a, s);
+    if (in == IN_FILL64 && out == OUT_STORE) return launch_nt<R, IN_FILL64, OUT_STORE>(nt, nblocks, nbatch, a, s);
+    if (in == IN_FILL64 && out == OUT_REAL) return launch_nt<R, IN_FILL64, OUT_REAL>(nt, nblocks, nbatch, a, s);
+    return hipErrorInvalidValue;
+}
+
+// Per batch member: fixed-order reduction of the per-block (sum, sum of squares, max|Im|)
+// partials -> mean, population variance, max|Im| (f90:1338-1343).
+__global__ __launch_bounds__(256) void stats_finalize_kernel(const double* __restrict__ partial, int nblocks,
+                                                             double ncell, double* __restrict__ stats) {
+    const int b = blockIdx.x;
+    const double* p = partial + 3 * (int64_t)b * nblocks;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nblocks; i += 256) {
+        v[0] += p[3 * i];
+        v[1] += p[3 * i + 1];
+        v[2] = fmax(v[2], p[3 * i + 2]);
+    }
+    double out[3];
+    const int ops[3] = {0, 0, 1};
+    block_reduce_write<256, 3>(v, ops, out);
+    if (threadIdx.x == 0) {
+        const double mean = out[0] / ncell;
+        double var = out[1] / ncell - mean * mean;
+        if (var < 0.0) var = 0.0;
+        stats[3 * b] = mean;
+        stats[3 * b + 1] = var;
+        stats[3 * b + 2] = out[2];
+    }
+}
+
+// (max|Im S|, max|Re S|, sneg, spos) over the spectrum blocks (f90:1277-1288).
+__global__ __launch_bounds__(256) void diag_finalize_kernel(const double* __restrict__ partial, int nblocks,
+                                                            double* __restrict__ diag4) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nblocks; i += 256) {
+        v[0] = fmax(v[0], partial[4 * i]);
+        v[1] = fmax(v[1], partial[4 * i + 1]);
+        v[2] += partial[4 * i + 2];
+        v[3] += partial[4 * i + 3];
+    }
+    const int ops[4] = {1, 1, 0, 0};
+    double out[4];
+    block_reduce_write<256, 4>(v, ops, out);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 4; ++c) diag4[c] = out[c];
+}
+
+hipError_t launch_stats_finalize(const double* partial, int nblocks, int nbatch, double ncell, double* stats,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(stats_finalize_kernel, dim3(nbatch), dim3(256), 0, s, partial, nblocks, ncell, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_finalize(const double* partial, int nblocks, double* diag4, hipStream_t s) {
+    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(256), 0, s, partial, nblocks, diag4);
+    return hipGetLastError();
+}
+
+}  // namespace ss3d
