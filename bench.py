@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Throughput of SPECSIM3D's unconditional spectral simulation on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a 256^3 output grid in a 512^3
+periodic grid; exponential model exp(-3r), sill 0.7 plus nugget 0.3, ranges 80/40/10, azimuth
+30, dip 15, ipad=1, igauss=1 (Gaussian spectral coefficients); 64 realizations per step.
+
+A step = one pass of the per-realization hot path over one batch: the 64 realizations (fill,
+inverse FFT passes, crop, statistics) written to HBM, inputs (the FP64 spectral density, built
+once before timing) already resident.  Timed: the FP32 batched path (the `value`) and the FP64
+drop-in path, each bracketed by device synchronisation.  Per-pass HIP events on the library's
+stream give the dominant kernel's duration for the roofline.  The CPU baseline is the
+reference program (oracle/_ref, built from the reference source) on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+STRUCTS = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]
+NUGGET = 0.3
+GRID = (256, 256, 256)
+ISEED = 69069
+IGAUSS = 1
+HBM_PEAK_GBS = 8000.0
+
+
+class Hip:
+    """Device allocations for the benchmark's output buffers (plumbing)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+
+    def malloc(self, n):
+        p = ctypes.c_void_p()
+        rc = self.L.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc({n}) failed: {rc}")
+        return p.value
+
+    def free(self, p):
+        self.L.hipFree(ctypes.c_void_p(p))
+
+
+def par_text(nreal, base):
+    s = STRUCTS[0]
+    return ("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d %d 1 %d\n%s\n1\n3\n0\n"
+            % (*GRID, NUGGET, s[0], *s[1:], ISEED, nreal, IGAUSS, base))
+
+
+def cpu_baseline(threads):
+    """Reference program on the host cores: 256^3 (512^3 periodic), one realization."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "specsim3d")
+    ncell = GRID[0] * GRID[1] * GRID[2]
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as wd:
+            with open(os.path.join(wd, "b.par"), "w") as fh:
+                fh.write(par_text(1, "b"))
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "b.par"], cwd=wd, capture_output=True, text=True,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=600)
+            wall = time.perf_counter() - t0
+        if r.returncode == 0:
+            tr = float(re.search(r"realizations\s+:\s+([\d.]+)", r.stdout).group(1))
+            ts = float(re.search(r"covariance\+spectrum:\s+([\d.]+)", r.stdout).group(1))
+            return {"value": ncell / tr / 1e9, "unit": "Gcells/s", "cores": threads, "kind": "reference",
+                    "sample": f"reference program (amdflang -O3 -fopenmp), 256^3 grid / 512^3 periodic, 1 realization: "
+                              f"time realizations {tr:.2f} s, covariance+spectrum {ts:.2f} s, wall {wall:.1f} s"}
+    # fallback: the oracle's C restatement, one thread, 128^3 grid (256^3 periodic)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+
+    t0 = time.perf_counter()
+    O.simulate(STRUCTS, NUGGET, 1, 1, 1, 128, 128, 128, 1, ISEED, 1, IGAUSS)
+    dt = time.perf_counter() - t0
+    return {"value": 128 ** 3 / dt / 1e9, "unit": "Gcells/s", "cores": 1, "kind": "port",
+            "sample": f"oracle C restatement, 128^3 grid / 256^3 periodic, 1 realization incl. spectrum: {dt:.2f} s"}
+
+
+def load_traffic(kind):
+    path = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    v = t.get(kind)
+    return None if v is None else v.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nreal", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Realizations are independent; see DESIGN.md for how the path would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    import specsim3d_amd as S
+
+    hip = Hip()
+    eng = S.Engine(0)
+    nx, ny, nz = GRID
+    grid = S.periodic_grid(STRUCTS, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    t0 = time.perf_counter()
+    spec = eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *grid)
+    eng.synchronize()
+    spec_ms = (time.perf_counter() - t0) * 1e3
+    nreal = args.nreal
+    ncell = nx * ny * nz
+    out32 = hip.malloc(nreal * ncell * 4)
+    stats = hip.malloc(nreal * 3 * 8)
+
+    def run(prec, out, step, count):
+        eng.realize_device(spec, ISEED, IGAUSS, step * nreal, count, nx, ny, nz, prec, out, stats)
+
+    # ---- FP32 batched path
+    for w in range(args.warmup):
+        run(S.FP32, out32, w, nreal)
+    eng.synchronize()
+    eng.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run(S.FP32, out32, args.warmup + k, nreal)
+    eng.synchronize()
+    t32 = time.perf_counter() - t0
+    eng.profile(False)
+    prof32 = eng.profile_read()
+    hip.free(out32)
+
+    # ---- FP64 drop-in path (same batch, FP64 output)
+    out64 = hip.malloc(nreal * ncell * 8)
+    run(S.FP64, out64, 0, nreal)
+    eng.synchronize()
+    eng.profile(True)
+    t0 = time.perf_counter()
+    for k in range(args.fp64_steps):
+        run(S.FP64, out64, 1 + k, nreal)
+    eng.synchronize()
+    t64 = time.perf_counter() - t0
+    eng.profile(False)
+    prof64 = eng.profile_read()
+    hip.free(out64)
+    hip.free(stats)
+    spec.close()
+    eng.close()
+
+    cells32 = nreal * ncell * args.steps
+    value = cells32 / t32 / 1e9
+    fp64_value = nreal * ncell * args.fp64_steps / t64 / 1e9
+
+    # dominant kernel of the measured (FP32) path
+    kind = max(prof32, key=lambda k: prof32[k][0])
+    ms, byt, nl = prof32[kind]
+    achieved = byt / (ms * 1e-3) / 1e9
+    passes = {k: {"ms_per_launch": v[0] / v[2], "GB_per_launch": v[1] / v[2] / 1e9,
+                  "GB_s": v[1] / (v[0] * 1e-3) / 1e9, "launches": v[2]} for k, v in {**prof32, **prof64}.items()}
+
+    line = {
+        "metric": "Gcells/s at 256^3 (FP32 batched vs FP64 plain); p50/p99 request latency",
+        "value": round(value, 3),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "256^3 grid (512^3 periodic), exponential sill 0.7 + nugget 0.3, ranges 80/40/10, "
+                               "az 30, dip 15, ipad=1, igauss=1, 64 realizations per step, FP32 output in HBM",
+                   "global_batch": nreal, "grid": list(GRID), "periodic_grid": list(grid), "parallelism": "none"},
+        "fp64_plain": {"value": round(fp64_value, 3), "unit": "Gcells/s", "steps": args.fp64_steps,
+                       "ms_per_realization": round(t64 / (args.fp64_steps * nreal) * 1e3, 4)},
+        "fp32_over_fp64": round(value / fp64_value, 2),
+        "spectrum_ms": round(spec_ms, 2),
+        "latency_ms": {"p50": None, "p99": None, "note": "serving loop not measured in this run"},
+        "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kind),
+                     "algorithmic_bytes_per_launch": byt / nl, "ms_per_launch": ms / nl},
+        "passes": passes,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(min(os.cpu_count() or 1, 16))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

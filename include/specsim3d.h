@@ -108,6 +108,16 @@ int ss3d_realize(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int ig
 int ss3d_realize_device(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int igauss, int64_t r0,
                         int nreal, int nx, int ny, int nz, int precision, void* dev_out, double* dev_stats);
 
+/* Per-pass timing with HIP events recorded on the context stream around every pass launch.
+ * Pass kinds: 0..2 spectrum x/y/z (FP64), 3..5 FP64 realization x/y/z, 6..8 FP32 realization
+ * z/y/x.  enable=1 clears and starts recording, enable=0 stops.  profile_read synchronises the
+ * stream and returns, per kind, total milliseconds, algorithmic HBM bytes (bytes each pass must
+ * read and write) and launch count.  No counterpart in the reference (its timers are the wall
+ * clocks printed at f90:1299-1300, 1350-1351). */
+#define SS3D_NPASS_KINDS 9
+int ss3d_context_profile(ss3d_context* ctx, int enable);
+int ss3d_context_profile_read(ss3d_context* ctx, double* ms, double* bytes, int64_t* launches, int nkinds);
+
 #ifdef __cplusplus
 }
 #endif

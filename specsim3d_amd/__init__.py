@@ -69,7 +69,13 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _dp, _dp, _dp]),
     "ss3d_realize_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "ss3d_context_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
 }
+
+NPASS_KINDS = 9
+PASS_KINDS = ["spectrum_x_fp64", "spectrum_y_fp64", "spectrum_z_fp64", "real_x_fp64", "real_y_fp64", "real_z_fp64",
+              "real_z_fp32", "real_y_fp32", "real_x_c2r_fp32"]
 
 
 def lib():
@@ -188,6 +194,18 @@ class Engine:
 
     def synchronize(self):
         check(lib().ss3d_context_synchronize(self.handle))
+
+    def profile(self, enable):
+        """Start (clears) or stop per-pass HIP-event timing on the context stream."""
+        check(lib().ss3d_context_profile(self.handle, 1 if enable else 0))
+
+    def profile_read(self):
+        """{kind: (ms, algorithmic bytes, launches)} for kinds with launches."""
+        ms = (ctypes.c_double * NPASS_KINDS)()
+        by = (ctypes.c_double * NPASS_KINDS)()
+        ln = (ctypes.c_int64 * NPASS_KINDS)()
+        check(lib().ss3d_context_profile_read(self.handle, ms, by, ln, NPASS_KINDS))
+        return {PASS_KINDS[k]: (ms[k], by[k], ln[k]) for k in range(NPASS_KINDS) if ln[k]}
 
     def close(self):
         if self.handle:

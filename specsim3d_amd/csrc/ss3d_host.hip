@@ -179,6 +179,22 @@ struct ss3d_context {
     DevBuf wa, wb, partial, stats, dout, rbase, diag;
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
+    // per-pass event timing (ss3d_context_profile)
+    struct ProfRec {
+        int kind;
+        hipEvent_t e0, e1;
+        double bytes;
+    };
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> event_pool;
+    ~ss3d_context() {
+        for (auto& r : prof) {
+            (void)hipEventDestroy(r.e0);
+            (void)hipEventDestroy(r.e1);
+        }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+    }
 };
 
 struct RngTables {
@@ -307,7 +323,44 @@ struct Pass {
     Shape sh;
     int nblocks;
     PassArgs a;
+    int kind;  // profiling kind (see ss3d_context_profile)
 };
+
+// Bytes a pass must move through HBM per batch member: the operands it reads (complex input,
+// or the amplitude it fills from) and the results it writes.
+double pass_bytes(const Pass& p) {
+    const PassArgs& a = p.a;
+    const double ec = p.prec == 64 ? 16.0 : 8.0, er = ec / 2;
+    const double n = double(int64_t(1) << a.logn);
+    const double cols = a.axis == 0 ? double(a.nrows) : double(a.n0) * a.nthird;
+    double in = 0, out = 0;
+    switch (p.in) {
+        case IN_LOAD: in = cols * n * ec; break;
+        case IN_LOADH: in = cols * a.hx * ec; break;
+        case IN_FILL64: in = cols * n * 8.0; break;
+        case IN_FILL32H: in = cols * n * 4.0; break;
+        case IN_FILLH: in = a.hx * 4.0; break;
+        default: break;
+    }
+    switch (p.out) {
+        case OUT_STORE: out = cols * a.n_out * ec; break;
+        case OUT_REAL: out = cols * a.n_out * er; break;
+        case OUT_SPEC: out = double(a.npx) * a.npy * a.npz * 8.0 + double(a.hx) * a.npy * a.npz * 4.0; break;
+        default: break;
+    }
+    return in + out;
+}
+
+hipEvent_t take_event(ss3d_context* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
 
 int run_pass(ss3d_context* c, Pass& p, int nbatch) {
     p.a.logp = p.sh.logp;
@@ -324,10 +377,21 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch) {
         SS3D_TRY(c->partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), c->stream));
         p.a.partial = c->partial.as<double>();
     }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profiling) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (!e0 || !e1) return fail(SS3D_EHIP, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(e0, c->stream));
+    }
     hipError_t e = p.prec == 64 ? launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream)
                                 : launch_pass32(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream);
     if (e != hipSuccess) return fail(SS3D_EHIP, "pass launch (prec %d in %d out %d nt %d): %s", p.prec, p.in, p.out,
                                      p.sh.nt, hipGetErrorString(e));
+    if (c->profiling) {
+        HIP_TRY(hipEventRecord(e1, c->stream));
+        c->prof.push_back({p.kind, e0, e1, pass_bytes(p) * nbatch});
+    }
     return SS3D_OK;
 }
 
@@ -476,7 +540,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     const int emax = 8192;
     const bool has_y = npy > 1, has_z = npz > 1;
     // x pass: covariance generated in registers, transformed along x (isign = -1)
-    Pass px{64, IN_COV, (has_y || has_z) ? OUT_STORE : OUT_SPEC, contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a};
+    Pass px{64, IN_COV, (has_y || has_z) ? OUT_STORE : OUT_SPEC, contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a, 0};
     px.a.axis = 0;
     px.a.logn = lx;
     px.a.nrows = npy * npz;
@@ -486,7 +550,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     px.a.tw = tw_ptr(c->tw64f, lx, 16);
     SS3D_TRY(run_pass(c, px, 1));
     Pass* last = &px;
-    Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a};
+    Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a, 1};
     if (has_y) {
         py.a.axis = 1;
         py.a.logn = ly;
@@ -501,7 +565,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
         SS3D_TRY(run_pass(c, py, 1));
         last = &py;
     }
-    Pass pz{64, IN_LOAD, OUT_SPEC, strided_shape(lz, npx, 128, 16, c->target_e64, emax), 0, a};
+    Pass pz{64, IN_LOAD, OUT_SPEC, strided_shape(lz, npx, 128, 16, c->target_e64, emax), 0, a, 2};
     if (has_z) {
         pz.a.axis = 2;
         pz.a.logn = lz;
@@ -528,6 +592,39 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     sp->diag.c0 = c0;
     if (diag) *diag = sp->diag;
     *out = sp.release();
+    return SS3D_OK;
+}
+
+int ss3d_context_profile(ss3d_context* c, int enable) {
+    if (!c) return fail(SS3D_EINVAL, "null context");
+    if (enable) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (auto& r : c->prof) {
+            c->event_pool.push_back(r.e0);
+            c->event_pool.push_back(r.e1);
+        }
+        c->prof.clear();
+    }
+    c->profiling = enable != 0;
+    return SS3D_OK;
+}
+
+int ss3d_context_profile_read(ss3d_context* c, double* ms, double* bytes, int64_t* launches, int nkinds) {
+    if (!c || !ms || !bytes || !launches || nkinds < 1) return fail(SS3D_EINVAL, "null argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < nkinds; ++k) {
+        ms[k] = 0;
+        bytes[k] = 0;
+        launches[k] = 0;
+    }
+    for (auto& r : c->prof) {
+        if (r.kind < 0 || r.kind >= nkinds) continue;
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, r.e0, r.e1));
+        ms[r.kind] += t;
+        bytes[r.kind] += r.bytes;
+        launches[r.kind] += 1;
+    }
     return SS3D_OK;
 }
 
@@ -593,7 +690,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         SS3D_TRY(c->wa.ensure((size_t)b1 * nb * 16, c->stream));
         SS3D_TRY(c->wb.ensure((size_t)std::max<int64_t>(b2, 1) * nb * 16, c->stream));
         p1 = Pass{64, IN_FILL64, (has_y || has_z) ? OUT_STORE : OUT_REAL,
-                  contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a};
+                  contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a, 3};
         p1.a.axis = 0;
         p1.a.logn = lx;
         p1.a.nrows = npy * npz;
@@ -611,7 +708,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         SS3D_TRY(run_pass(c, p1, nb));
         last = &p1;
         if (has_y) {
-            p2 = Pass{64, IN_LOAD, has_z ? OUT_STORE : OUT_REAL, strided_shape(ly, nx, 128, 16, c->target_e64, emax), 0, a};
+            p2 = Pass{64, IN_LOAD, has_z ? OUT_STORE : OUT_REAL, strided_shape(ly, nx, 128, 16, c->target_e64, emax), 0, a, 4};
             p2.a.axis = 1;
             p2.a.logn = ly;
             p2.a.n0 = nx;
@@ -630,7 +727,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             last = &p2;
         }
         if (has_z) {
-            p3 = Pass{64, IN_LOAD, OUT_REAL, strided_shape(lz, nx, 128, 16, c->target_e64, emax), 0, a};
+            p3 = Pass{64, IN_LOAD, OUT_REAL, strided_shape(lz, nx, 128, 16, c->target_e64, emax), 0, a, 5};
             p3.a.axis = 2;
             p3.a.logn = lz;
             p3.a.n0 = nx;
@@ -658,7 +755,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         const void* prev = nullptr;
         int64_t prev_b = 0;
         if (has_z) {
-            p1 = Pass{32, IN_FILL32H, OUT_STORE, strided_shape(lz, hx, 128, 8, c->target_e32, emax), 0, a};
+            p1 = Pass{32, IN_FILL32H, OUT_STORE, strided_shape(lz, hx, 128, 8, c->target_e32, emax), 0, a, 6};
             p1.a.axis = 2;
             p1.a.logn = lz;
             p1.a.n0 = hx;
@@ -674,7 +771,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             prev_b = bz;
         }
         if (has_y) {
-            p2 = Pass{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, strided_shape(ly, hx, 128, 8, c->target_e32, emax), 0, a};
+            p2 = Pass{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, strided_shape(ly, hx, 128, 8, c->target_e32, emax), 0, a, 7};
             p2.a.axis = 1;
             p2.a.logn = ly;
             p2.a.n0 = hx;
@@ -693,7 +790,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             prev = c->wb.p;
             prev_b = by;
         }
-        p3 = Pass{32, prev ? IN_LOADH : IN_FILLH, OUT_REAL, contig_shape(lx, (int64_t)ny * nz, c->target_e32, emax), 0, a};
+        p3 = Pass{32, prev ? IN_LOADH : IN_FILLH, OUT_REAL, contig_shape(lx, (int64_t)ny * nz, c->target_e32, emax), 0, a, 8};
         p3.a.axis = 0;
         p3.a.logn = lx;
         p3.a.nrows = ny * nz;
