@@ -1,0 +1,109 @@
+!=======================================================================
+! ss3d_capi -- ISO_C_BINDING interface of libspecsim3d.so (include/specsim3d.h).
+!
+! The Fortran driver (specsim3d.f90) reaches the HIP engine only through these interfaces:
+! plain C types, handles as c_ptr, integer return codes (0 = success, message from
+! ss3d_last_error).
+!=======================================================================
+module ss3d_capi
+  use iso_c_binding
+  implicit none
+  private
+  public :: ss3d_structure_t, ss3d_spec_diag_t, SS3D_FP64, SS3D_FP32
+  public :: ss3d_context_create, ss3d_context_destroy, ss3d_periodic_grid, ss3d_spectrum_create, &
+            ss3d_spectrum_destroy, ss3d_realize, ss3d_draws_per_realization, ss3d_check
+
+  integer(c_int), parameter :: SS3D_FP64 = 0, SS3D_FP32 = 1
+
+  ! ss3d_structure: scalar members of the reference's `type structure`
+  type, bind(C) :: ss3d_structure_t
+    integer(c_int32_t) :: itype = 1
+    integer(c_int32_t) :: pad_ = 0
+    real(c_double) :: sill = 0, ahmax = 1, ahmin = 1, avert = 1, az = 0, dip = 0, plg = 0
+  end type
+
+  type, bind(C) :: ss3d_spec_diag_t
+    real(c_double) :: max_imag = 0, max_real = 0, c0 = 0, spos = 0, sneg = 0
+  end type
+
+  interface
+    integer(c_int) function ss3d_context_create(device, ctx) bind(C, name='ss3d_context_create')
+      import :: c_int, c_ptr
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function ss3d_context_destroy(ctx) bind(C, name='ss3d_context_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function ss3d_periodic_grid(st, nst, dx, dy, dz, nx, ny, nz, ipad, npx, npy, npz) &
+        bind(C, name='ss3d_periodic_grid')
+      import :: c_int, c_double, ss3d_structure_t
+      type(ss3d_structure_t), intent(in) :: st(*)
+      integer(c_int), value :: nst, nx, ny, nz, ipad
+      real(c_double), value :: dx, dy, dz
+      integer(c_int), intent(out) :: npx, npy, npz
+    end function
+
+    integer(c_int) function ss3d_spectrum_create(ctx, st, nst, nugget, dx, dy, dz, npx, npy, npz, spec, diag) &
+        bind(C, name='ss3d_spectrum_create')
+      import :: c_int, c_double, c_ptr, ss3d_structure_t, ss3d_spec_diag_t
+      type(c_ptr), value :: ctx
+      type(ss3d_structure_t), intent(in) :: st(*)
+      integer(c_int), value :: nst, npx, npy, npz
+      real(c_double), value :: nugget, dx, dy, dz
+      type(c_ptr), intent(out) :: spec
+      type(ss3d_spec_diag_t), intent(out) :: diag
+    end function
+
+    integer(c_int) function ss3d_spectrum_destroy(spec) bind(C, name='ss3d_spectrum_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: spec
+    end function
+
+    integer(c_int64_t) function ss3d_draws_per_realization(npx, npy, npz, igauss) &
+        bind(C, name='ss3d_draws_per_realization')
+      import :: c_int, c_int64_t
+      integer(c_int), value :: npx, npy, npz, igauss
+    end function
+
+    integer(c_int) function ss3d_realize(ctx, spec, iseed, igauss, r0, nreal, nx, ny, nz, precision, host_out, &
+        mean, var, maximag) bind(C, name='ss3d_realize')
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, spec, host_out
+      integer(c_int), value :: iseed, igauss, nreal, nx, ny, nz, precision
+      integer(c_int64_t), value :: r0
+      real(c_double), intent(out) :: mean(*), var(*), maximag(*)
+    end function
+
+    integer(c_int) function ss3d_last_error_c(buf, length) bind(C, name='ss3d_last_error')
+      import :: c_int, c_int64_t, c_char
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int64_t), value :: length
+    end function
+  end interface
+
+contains
+
+  ! Stops the program with the library's message when rc is not 0.
+  subroutine ss3d_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char) :: buf(512)
+    character(len=512) :: msg
+    integer :: i, rc2
+    if (rc == 0) return
+    buf = c_null_char
+    rc2 = ss3d_last_error_c(buf, int(size(buf), c_int64_t))
+    msg = ''
+    do i = 1, size(buf)
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+    write(*,'(a,a,a,i0,a,a)') ' ', what, ' failed (code ', rc, '): ', trim(msg)
+    stop 1
+  end subroutine ss3d_check
+
+end module ss3d_capi
