@@ -125,6 +125,9 @@ def main():
     spec_ms = (time.perf_counter() - t0) * 1e3
     nreal = args.nreal
     ncell = nx * ny * nz
+    t0 = time.perf_counter()
+    plan = eng.autotune(spec, nx, ny, nz, IGAUSS, reps=3)
+    tune_s = time.perf_counter() - t0
     out32 = hip.malloc(nreal * ncell * 4)
     stats = hip.malloc(nreal * 3 * 8)
 
@@ -193,6 +196,7 @@ def main():
                        "ms_per_realization": round(t64 / (args.fp64_steps * nreal) * 1e3, 4)},
         "fp32_over_fp64": round(value / fp64_value, 2),
         "spectrum_ms": round(spec_ms, 2),
+        "autotune": {"largest_radix_log2": plan, "seconds": round(tune_s, 2)},
         "latency_ms": {"p50": None, "p99": None, "note": "serving loop not measured in this run"},
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kind),

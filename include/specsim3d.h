@@ -118,6 +118,15 @@ int ss3d_realize_device(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed,
 int ss3d_context_profile(ss3d_context* ctx, int enable);
 int ss3d_context_profile_read(ss3d_context* ctx, double* ms, double* bytes, int64_t* launches, int nkinds);
 
+/* Autotuner of the FP32 passes (no counterpart in the reference): times every radix plan of
+ * each FP32 pass (largest radix 8, 16 or 32) on this spectrum's grid and crop with `reps`
+ * timed batches per plan, and keeps the fastest for later calls on the same context.
+ * tuning_get returns the plan (log2 of the largest radix) of pass kind 6..8 and length
+ * 2^logn; tuning_set pins one (e.g. a plan stored from an earlier run). */
+int ss3d_autotune(ss3d_context* ctx, const ss3d_spectrum* spec, int nx, int ny, int nz, int igauss, int reps);
+int ss3d_tuning_get(const ss3d_context* ctx, int kind, int logn);
+int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,9 @@ SIGNATURES = {
     "ss3d_realize_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "ss3d_context_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ss3d_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ss3d_tuning_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "ss3d_tuning_set": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
 }
 
@@ -194,6 +197,17 @@ class Engine:
 
     def synchronize(self):
         check(lib().ss3d_context_synchronize(self.handle))
+
+    def autotune(self, spec, nx, ny, nz, igauss, reps=3):
+        """Pick the fastest radix plan of each FP32 pass for this grid; returns {kind: rm}."""
+        check(lib().ss3d_autotune(self.handle, spec.handle, nx, ny, nz, igauss, reps))
+        npx, npy, npz = spec.grid
+        lx, ly, lz = (int(v).bit_length() - 1 for v in (npx, npy, npz))
+        kinds = {"real_z_fp32": (6, lz), "real_y_fp32": (7, ly), "real_x_c2r_fp32": (8, max(lx - 1, 0))}
+        return {k: lib().ss3d_tuning_get(self.handle, *v) for k, v in kinds.items()}
+
+    def tuning_set(self, kind, logn, rm):
+        check(lib().ss3d_tuning_set(self.handle, kind, logn, rm))
 
     def profile(self, enable):
         """Start (clears) or stop per-pass HIP-event timing on the context stream."""

@@ -26,11 +26,14 @@ ARCH = "gfx950"
 # -ffp-contract=off: FP64 kernels and host model arithmetic round every product and sum
 # separately, as the reference build does (no FMA on its x86-64 baseline target).
 # packed-fp32-ops disabled: packed FP32 arithmetic is not used anywhere in this engine.
-COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+COMMON = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 UNITS = {
     "ss3d_k64.hip": ["-ffp-contract=off"],
     "ss3d_k32.hip": [],
+    "ss3d_k32_r3.hip": [],
+    "ss3d_k32_r4.hip": [],
+    "ss3d_k32_r5.hip": [],
     "ss3d_host.hip": ["-ffp-contract=off"],
 }
 

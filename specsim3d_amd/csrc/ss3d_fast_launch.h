@@ -1,0 +1,72 @@
+// ss3d_fast_launch.h -- launchers of the FP32 passes for one maximum radix (RM), instantiated
+// by ss3d_k32_r{3,4,5}.hip so that the variants compile in parallel.
+#pragma once
+
+#include "ss3d_fast.h"
+
+namespace ss3d {
+namespace fastlaunch {
+
+constexpr int strided_P(int L) { return L <= 7 ? 64 : (8192 >> L); }
+
+template <typename K>
+hipError_t launch_k(K k, size_t lds, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s, bool& attr) {
+    if (lds > 65536 && !attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, dim3(nblocks, nbatch), dim3(nt), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int L, int IN, int AXIS, int RM>
+hipError_t strided_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    using C = fast::StridedCfg<L, strided_P(L), RM>;
+    static bool attr = false;
+    return launch_k(fast::strided_kernel<L, strided_P(L), IN, AXIS, RM>, C::lds_elems() * sizeof(float2), C::NT,
+                    nblocks, nbatch, a, s, attr);
+}
+
+template <int L, int IN, int RM>
+hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    using C = fast::C2RCfg<L, RM>;
+    static bool attr = false;
+    return launch_k(fast::c2r_kernel<L, IN, RM>, C::lds_elems() * sizeof(float2), C::NT, nblocks, nbatch, a, s, attr);
+}
+
+template <int RM, int IN, int AXIS, int... Ls>
+hipError_t strided_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
+                            std::integer_sequence<int, Ls...>) {
+    hipError_t e = hipErrorInvalidValue;
+    ((L == Ls + 1 ? (e = strided_L<Ls + 1, IN, AXIS, RM>(nblocks, nbatch, a, s), 0) : 0), ...);
+    return e;
+}
+
+template <int RM, int IN, int... Ls>
+hipError_t c2r_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
+                        std::integer_sequence<int, Ls...>) {
+    hipError_t e = hipErrorInvalidValue;
+    ((L == Ls ? (e = c2r_L<Ls, IN, RM>(nblocks, nbatch, a, s), 0) : 0), ...);
+    return e;
+}
+
+template <int RM>
+hipError_t strided_rm(int L, int in_mode, int axis, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    using seq = std::make_integer_sequence<int, kMaxFastLog>;
+    if (in_mode == IN_LOAD) return strided_dispatch<RM, IN_LOAD, 1>(L, nblocks, nbatch, a, s, seq{});
+    if (in_mode == IN_FILL32H && axis == 2) return strided_dispatch<RM, IN_FILL32H, 2>(L, nblocks, nbatch, a, s, seq{});
+    if (in_mode == IN_FILL32H && axis == 1) return strided_dispatch<RM, IN_FILL32H, 1>(L, nblocks, nbatch, a, s, seq{});
+    return hipErrorInvalidValue;
+}
+
+template <int RM>
+hipError_t c2r_rm(int L, int in_mode, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    using seq = std::make_integer_sequence<int, kMaxFastLog>;
+    if (in_mode == IN_LOADH) return c2r_dispatch<RM, IN_LOADH>(L, nblocks, nbatch, a, s, seq{});
+    if (in_mode == IN_FILLH) return c2r_dispatch<RM, IN_FILLH>(L, nblocks, nbatch, a, s, seq{});
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fastlaunch
+}  // namespace ss3d

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -176,6 +177,8 @@ struct ss3d_context {
     hipStream_t stream = nullptr;
     // twiddles for lengths 2^1..2^kMaxLogN: entry k of length n at offset (n/2 - 1) + k
     DevBuf tw64f, tw64i, tw32i;
+    // FP32 W_n tables for n = 2^0..2^kMaxLogN: entry m = exp(+2 pi i m/n) at offset (n - 1) + m
+    DevBuf tw32n;
     DevBuf wa, wb, partial, stats, dout, rbase, diag;
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
@@ -187,6 +190,8 @@ struct ss3d_context {
     };
     bool profiling = false;
     std::vector<ProfRec> prof;
+    // autotuned radix plan of the FP32 passes: (profiling kind, log2 length) -> largest radix log
+    std::map<std::pair<int, int>, int> tune;
     std::vector<hipEvent_t> event_pool;
     ~ss3d_context() {
         for (auto& r : prof) {
@@ -236,6 +241,18 @@ int build_twiddles(ss3d_context* c) {
             i32[2 * (off + k) + 1] = (float)sn;
         }
     }
+    std::vector<float> wn(2 * ((size_t(2) << kMaxLogN) - 1));
+    for (int L = 0; L <= kMaxLogN; ++L) {
+        const int n = 1 << L;
+        const size_t off = (size_t(1) << L) - 1;
+        for (int m = 0; m < n; ++m) {
+            const double ang = 2.0 * kPi * (double)m / (double)n;
+            wn[2 * (off + m)] = (float)std::cos(ang);
+            wn[2 * (off + m) + 1] = (float)std::sin(ang);
+        }
+    }
+    SS3D_TRY(c->tw32n.ensure(wn.size() * 4, c->stream));
+    HIP_TRY(hipMemcpy(c->tw32n.p, wn.data(), wn.size() * 4, hipMemcpyHostToDevice));
     SS3D_TRY(c->tw64f.ensure(f.size() * 8, c->stream));
     SS3D_TRY(c->tw64i.ensure(iv.size() * 8, c->stream));
     SS3D_TRY(c->tw32i.ensure(i32.size() * 4, c->stream));
@@ -323,7 +340,9 @@ struct Pass {
     Shape sh;
     int nblocks;
     PassArgs a;
-    int kind;  // profiling kind (see ss3d_context_profile)
+    int kind;      // profiling kind (see ss3d_context_profile)
+    int fast = 0;  // FP32 fast kernels: 1 strided, 2 C2R x pass, 3 one-point x axis
+    int rm = 5;    // FP32 fast kernels: largest radix 2^rm
 };
 
 // Bytes a pass must move through HBM per batch member: the operands it reads (complex input,
@@ -333,6 +352,8 @@ double pass_bytes(const Pass& p) {
     const double ec = p.prec == 64 ? 16.0 : 8.0, er = ec / 2;
     const double n = double(int64_t(1) << a.logn);
     const double cols = a.axis == 0 ? double(a.nrows) : double(a.n0) * a.nthird;
+    if (p.fast == 2 || p.fast == 3)  // C2R x pass: half rows in, real cropped rows out
+        return (p.in == IN_LOADH || p.fast == 3 ? cols * a.hx * ec : a.hx * 4.0) + cols * a.n_out * er;
     double in = 0, out = 0;
     switch (p.in) {
         case IN_LOAD: in = cols * n * ec; break;
@@ -364,7 +385,10 @@ hipEvent_t take_event(ss3d_context* c) {
 
 int run_pass(ss3d_context* c, Pass& p, int nbatch) {
     p.a.logp = p.sh.logp;
-    const int P = 1 << p.sh.logp;
+    int P = 1 << p.sh.logp;
+    if (p.fast == 1) P = fast_strided_tile(p.a.logn);
+    if (p.fast == 2) P = fast_c2r_rows(p.a.logn, p.rm);
+    if (p.fast == 3) P = 256;
     if (p.a.axis == 0) {
         p.nblocks = (int)((p.a.nrows + P - 1) / P);
     } else {
@@ -384,8 +408,15 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch) {
         if (!e0 || !e1) return fail(SS3D_EHIP, "hipEventCreate failed");
         HIP_TRY(hipEventRecord(e0, c->stream));
     }
-    hipError_t e = p.prec == 64 ? launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream)
-                                : launch_pass32(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream);
+    hipError_t e;
+    if (p.fast == 1)
+        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.nblocks, nbatch, p.a, c->stream);
+    else if (p.fast == 2)
+        e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, c->stream);
+    else if (p.fast == 3)
+        e = launch_fast_x1(p.nblocks, nbatch, p.a, c->stream);
+    else
+        e = launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream);
     if (e != hipSuccess) return fail(SS3D_EHIP, "pass launch (prec %d in %d out %d nt %d): %s", p.prec, p.in, p.out,
                                      p.sh.nt, hipGetErrorString(e));
     if (c->profiling) {
@@ -656,6 +687,11 @@ int ss3d_spectrum_amplitude(const ss3d_spectrum* sp, double* host_amp) {
 // ---------------------------------------------------------------- realizations
 namespace {
 
+int tuned_rm(const ss3d_context* c, int kind, int L) {
+    auto it = c->tune.find({kind, L});
+    return it == c->tune.end() ? 5 : it->second;
+}
+
 // Enqueue realizations r0..r0+nb-1 into dout (device) and their statistics into dstats.
 int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nb, int nx, int ny,
                   int nz, int prec, void* dout, double* dstats) {
@@ -747,15 +783,18 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         }
     } else {
         // FP32: z -> y over the half spectrum kx <= npx/2, then the C2R x pass
-        const int emax = 16384;
-        if (lx > 14 || ly > 14 || lz > 14) return fail(SS3D_ETOOBIG, "periodic axis longer than 16384");
+        if (lx > kMaxFastLog || ly > kMaxFastLog || lz > kMaxFastLog)
+            return fail(SS3D_ETOOBIG, "periodic axis longer than %d", 1 << kMaxFastLog);
         const int64_t bz = (int64_t)hx * npy * nz, by = (int64_t)hx * ny * nz;
         SS3D_TRY(c->wa.ensure((size_t)std::max<int64_t>(bz, 1) * nb * 8, c->stream));
         SS3D_TRY(c->wb.ensure((size_t)std::max<int64_t>(by, 1) * nb * 8, c->stream));
+        const float2* tw32 = c->tw32n.as<float2>();
+        auto wtab = [&](int L) { return static_cast<const void*>(tw32 + ((size_t(1) << L) - 1)); };
         const void* prev = nullptr;
         int64_t prev_b = 0;
         if (has_z) {
-            p1 = Pass{32, IN_FILL32H, OUT_STORE, strided_shape(lz, hx, 128, 8, c->target_e32, emax), 0, a, 6};
+            p1 = Pass{32, IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 6, 1};
+            p1.rm = tuned_rm(c, 6, lz);
             p1.a.axis = 2;
             p1.a.logn = lz;
             p1.a.n0 = hx;
@@ -765,13 +804,14 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p1.a.out_sA = (int64_t)hx * npy;
             p1.a.out_sT = hx;
             p1.a.out_bstride = bz;
-            p1.a.tw = tw_ptr(c->tw32i, lz, 8);
+            p1.a.tw = wtab(lz);
             SS3D_TRY(run_pass(c, p1, nb));
             prev = c->wa.p;
             prev_b = bz;
         }
         if (has_y) {
-            p2 = Pass{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, strided_shape(ly, hx, 128, 8, c->target_e32, emax), 0, a, 7};
+            p2 = Pass{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 7, 1};
+            p2.rm = tuned_rm(c, 7, ly);
             p2.a.axis = 1;
             p2.a.logn = ly;
             p2.a.n0 = hx;
@@ -785,14 +825,16 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p2.a.out_sA = hx;
             p2.a.out_sT = (int64_t)hx * ny;
             p2.a.out_bstride = by;
-            p2.a.tw = tw_ptr(c->tw32i, ly, 8);
+            p2.a.tw = wtab(ly);
             SS3D_TRY(run_pass(c, p2, nb));
             prev = c->wb.p;
             prev_b = by;
         }
-        p3 = Pass{32, prev ? IN_LOADH : IN_FILLH, OUT_REAL, contig_shape(lx, (int64_t)ny * nz, c->target_e32, emax), 0, a, 8};
+        // x pass: npx >= 2 -> C2R on complex length npx/2; npx = 1 -> real part of each row
+        p3 = Pass{32, prev ? IN_LOADH : IN_FILLH, OUT_REAL, Shape{0, 0}, 0, a, 8, npx >= 2 ? 2 : 3};
+        p3.rm = tuned_rm(c, 8, npx >= 2 ? lx - 1 : 0);
         p3.a.axis = 0;
-        p3.a.logn = lx;
+        p3.a.logn = npx >= 2 ? lx - 1 : 0;
         p3.a.nrows = ny * nz;
         p3.a.n_out = nx;
         p3.a.in = prev;
@@ -801,7 +843,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         p3.a.out = dout;
         p3.a.out_row = nx;
         p3.a.out_bstride = ncrop;
-        p3.a.tw = tw_ptr(c->tw32i, lx, 8);
+        p3.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
         SS3D_TRY(run_pass(c, p3, nb));
         last = &p3;
     }
@@ -886,6 +928,56 @@ int ss3d_realize(ss3d_context* c, const ss3d_spectrum* spc, int iseed, int igaus
             if (maximag) maximag[done + i] = hs[3 * i + 2];
         }
     }
+    return SS3D_OK;
+}
+
+// Autotuner: for each FP32 pass of this (grid, crop, igauss), time every radix plan
+// (largest radix 8, 16, 32) on the actual shape and keep the fastest.  Timing uses the
+// per-pass events of the profiler on `reps` batches per plan; results are stored in the
+// context and used by every later FP32 call with the same pass lengths.
+int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int nz, int igauss, int reps) {
+    SS3D_TRY(check_realize(c, spc, igauss, 0, 1, nx, ny, nz, SS3D_FP32));
+    HIP_TRY(hipSetDevice(c->device));
+    ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
+    const int lx = ilog2(sp->npx), ly = ilog2(sp->npy), lz = ilog2(sp->npz);
+    const int Lk[3] = {lz, ly, sp->npx >= 2 ? lx - 1 : 0};
+    const int nb = batch_size(sp, nx, ny, nz, SS3D_FP32, 64);
+    const int64_t ncrop = (int64_t)nx * ny * nz;
+    SS3D_TRY(c->dout.ensure((size_t)nb * ncrop * 4, c->stream));
+    SS3D_TRY(c->stats.ensure((size_t)nb * 3 * sizeof(double), c->stream));
+    const bool was = c->profiling;
+    double best[3] = {1e30, 1e30, 1e30};
+    int pick[3] = {5, 5, 5};
+    for (int rm = 3; rm <= 5; ++rm) {
+        for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = rm;
+        SS3D_TRY(enqueue_batch(c, sp, 1, igauss, 0, nb, nx, ny, nz, SS3D_FP32, c->dout.p, c->stats.as<double>()));
+        SS3D_TRY(ss3d_context_profile(c, 1));
+        for (int r = 0; r < std::max(1, reps); ++r)
+            SS3D_TRY(enqueue_batch(c, sp, 1, igauss, 0, nb, nx, ny, nz, SS3D_FP32, c->dout.p, c->stats.as<double>()));
+        double ms[SS3D_NPASS_KINDS], by[SS3D_NPASS_KINDS];
+        int64_t nl[SS3D_NPASS_KINDS];
+        SS3D_TRY(ss3d_context_profile_read(c, ms, by, nl, SS3D_NPASS_KINDS));
+        for (int k = 0; k < 3; ++k)
+            if (nl[6 + k] && ms[6 + k] < best[k]) {
+                best[k] = ms[6 + k];
+                pick[k] = rm;
+            }
+    }
+    SS3D_TRY(ss3d_context_profile(c, was ? 1 : 0));
+    for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = pick[k];
+    return SS3D_OK;
+}
+
+int ss3d_tuning_get(const ss3d_context* c, int kind, int logn) {
+    if (!c) return -1;
+    return tuned_rm(c, kind, logn);
+}
+
+int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
+    if (!c) return fail(SS3D_EINVAL, "null context");
+    if (kind < 6 || kind > 8 || logn < 0 || logn > kMaxFastLog || rm < 3 || rm > 5)
+        return fail(SS3D_EINVAL, "tuning entry out of range (kind 6..8, logn 0..%d, rm 3..5)", kMaxFastLog);
+    c->tune[{kind, logn}] = rm;
     return SS3D_OK;
 }
 

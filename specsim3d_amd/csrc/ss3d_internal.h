@@ -8,7 +8,8 @@
 namespace ss3d {
 
 constexpr int kMaxStruct = 10;     // maxst, src/specsim3d.f90:93
-constexpr int kMaxLogN = 14;       // longest axis the LDS-resident passes take: 2^14 (FP32), 2^13 (FP64)
+constexpr int kMaxLogN = 14;       // twiddle tables cover lengths up to 2^14
+constexpr int kMaxFastLog = 13;    // longest axis of the FP32 passes: 2^13
 constexpr uint32_t kPmM = 2147483647u;  // Park-Miller modulus 2^31-1 (f90:193)
 constexpr uint32_t kPmA = 16807u;       // Park-Miller multiplier (f90:193)
 constexpr int kEPT = 16;           // complex elements per thread in every FFT pass
@@ -69,13 +70,22 @@ enum OutMode { OUT_STORE = 0, OUT_SPEC = 1, OUT_REAL = 2 };
 // (64..1024), nbatch: batch members (grid y).  Return hipSuccess or the launch error.
 hipError_t launch_pass64(int in_mode, int out_mode, int nt, int nblocks, int nbatch, const PassArgs& a,
                          hipStream_t s);
-hipError_t launch_pass32(int in_mode, int out_mode, int nt, int nblocks, int nbatch, const PassArgs& a,
-                         hipStream_t s);
 // Reduce per-block partials: statistics -> (mean, var, max|Im|) per batch member, and
 // spectrum diagnostics -> (max|Im S|, max|Re S|, sneg, spos).
 hipError_t launch_stats_finalize(const double* partial, int nblocks, int nbatch, double ncell, double* stats,
                                  hipStream_t s);
 hipError_t launch_diag_finalize(const double* partial, int nblocks, double* diag4, hipStream_t s);
+
+// FP32 realization passes (ss3d_fast.h).  `tw` of a strided pass points at the W_n table of
+// its length (entry m = exp(+2 pi i m/n), FP32); a C2R pass gets the W_N table of its complex
+// length N = npx/2, immediately followed by the W_2N table.
+// rm: largest radix 2^rm of the pass plan (3, 4 or 5).
+int fast_strided_tile(int L);        // columns per block
+int fast_c2r_rows(int L, int rm);    // rows per block
+hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int nblocks, int nbatch, const PassArgs& a,
+                               hipStream_t s);
+hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
+hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 
 // LDS bytes a pass with nt threads uses (dynamic shared memory).
 inline size_t pass_lds_bytes(int nt, size_t elem_bytes) {

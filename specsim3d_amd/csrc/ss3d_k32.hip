@@ -1,16 +1,47 @@
-// ss3d_k32.hip -- FP32 instantiations of the pass kernel: the fast realization path
-// (half-spectrum z and y passes, C2R x pass with FP64 statistics).
-#include "ss3d_pass.h"
+// ss3d_k32.hip -- dispatch of the FP32 realization passes over their radix variants.
+#include "ss3d_fast_launch.h"
 
 namespace ss3d {
 
-hipError_t launch_pass32(int in, int out, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
-    using R = float;
-    if (in == IN_FILL32H && out == OUT_STORE) return launch_nt<R, IN_FILL32H, OUT_STORE>(nt, nblocks, nbatch, a, s);
-    if (in == IN_LOAD && out == OUT_STORE) return launch_nt<R, IN_LOAD, OUT_STORE>(nt, nblocks, nbatch, a, s);
-    if (in == IN_LOADH && out == OUT_REAL) return launch_nt<R, IN_LOADH, OUT_REAL>(nt, nblocks, nbatch, a, s);
-    if (in == IN_FILLH && out == OUT_REAL) return launch_nt<R, IN_FILLH, OUT_REAL>(nt, nblocks, nbatch, a, s);
-    return hipErrorInvalidValue;
+hipError_t launch_fast_strided_r3(int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_strided_r4(int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_strided_r5(int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_c2r_r3(int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_c2r_r4(int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_c2r_r5(int, int, int, int, const PassArgs&, hipStream_t);
+
+int fast_strided_tile(int L) { return fastlaunch::strided_P(L); }
+
+int fast_c2r_rows(int L, int rm) {
+    const int el = rm == 3 ? fast::eptlog(L, 3) : rm == 4 ? fast::eptlog(L, 4) : fast::eptlog(L, 5);
+    const int g = (1 << L) >> el;
+    return g >= 256 ? 1 : 256 / g;
+}
+
+hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int nblocks, int nbatch, const PassArgs& a,
+                               hipStream_t s) {
+    if (L < 1 || L > kMaxFastLog) return hipErrorInvalidValue;
+    switch (rm) {
+        case 3: return launch_fast_strided_r3(L, in_mode, axis, nblocks, nbatch, a, s);
+        case 4: return launch_fast_strided_r4(L, in_mode, axis, nblocks, nbatch, a, s);
+        case 5: return launch_fast_strided_r5(L, in_mode, axis, nblocks, nbatch, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    if (L < 0 || L > kMaxFastLog - 1) return hipErrorInvalidValue;
+    switch (rm) {
+        case 3: return launch_fast_c2r_r3(L, in_mode, nblocks, nbatch, a, s);
+        case 4: return launch_fast_c2r_r4(L, in_mode, nblocks, nbatch, a, s);
+        case 5: return launch_fast_c2r_r5(L, in_mode, nblocks, nbatch, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(fast::x1_kernel<0>, dim3(nblocks, nbatch), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace ss3d

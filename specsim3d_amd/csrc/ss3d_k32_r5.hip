@@ -1,0 +1,11 @@
+// ss3d_k32_r5.hip -- FP32 passes with radices up to 2^5 (see ss3d_fast_launch.h).
+#include "ss3d_fast_launch.h"
+
+namespace ss3d {
+hipError_t launch_fast_strided_r5(int L, int in_mode, int axis, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    return fastlaunch::strided_rm<5>(L, in_mode, axis, nblocks, nbatch, a, s);
+}
+hipError_t launch_fast_c2r_r5(int L, int in_mode, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    return fastlaunch::c2r_rm<5>(L, in_mode, nblocks, nbatch, a, s);
+}
+}  // namespace ss3d

@@ -136,3 +136,35 @@ def test_config2_full_size_fp32_vs_fp64(eng):
     assert mi64[0] < 1e-10
     assert 0.5 < v64[0] < 1.5 and 0.5 < v32[1] < 1.5
     assert not np.array_equal(f32[0], f32[1])
+
+
+@pytest.mark.parametrize("rm", [3, 4, 5])
+@pytest.mark.parametrize("grid", [(64, 48, 20), (100, 30, 1), (40, 1, 1), (24, 20, 12)])
+def test_fp32_radix_variants(eng, rm, grid):
+    """Every radix plan the autotuner may pick gives the FP64 field of the same draws."""
+    say(f"[radix] rm {rm} grid {grid}")
+    sts = [(2, 0.7, 8, 4, 2, 30, 15, 0)]
+    npg = S.periodic_grid(sts, 1, 1, 1, *grid, 1)
+    lx, ly, lz = (v.bit_length() - 1 for v in npg)
+    for kind, L in ((6, lz), (7, ly), (8, max(lx - 1, 0))):
+        eng.tuning_set(kind, L, rm)
+    sp = eng.spectrum(sts, 0.3, 1, 1, 1, *npg)
+    f64, m64, v64, _ = eng.realize(sp, 4711, 1, 0, 2, *grid, S.FP64)
+    f32, m32, v32, _ = eng.realize(sp, 4711, 1, 0, 2, *grid, S.FP32)
+    sp.close()
+    for kind, L in ((6, lz), (7, ly), (8, max(lx - 1, 0))):
+        eng.tuning_set(kind, L, 5)
+    assert rel_rms(f32.astype(np.float64), f64) < 1e-5
+    assert np.allclose(m32, m64, atol=1e-5) and np.allclose(v32, v64, rtol=1e-5)
+
+
+def test_autotune_returns_a_plan(eng):
+    sts = [(1, 1.0, 20, 10, 5, 45, 0, 0)]
+    grid = S.periodic_grid(sts, 1, 1, 1, 64, 64, 16, 1)
+    sp = eng.spectrum(sts, 0.0, 1, 1, 1, *grid)
+    plan = eng.autotune(sp, 64, 64, 16, 0, reps=1)
+    a, _, _, _ = eng.realize(sp, 69069, 0, 0, 1, 64, 64, 16, S.FP32)
+    sp.close()
+    assert set(plan.values()) <= {3, 4, 5}
+    o = O.simulate(sts, 0.0, 1, 1, 1, 64, 64, 16, 1, 69069, 1, 0)
+    assert rel_rms(a.astype(np.float64), o["res"]) < 1e-5
