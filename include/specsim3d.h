@@ -127,6 +127,12 @@ int ss3d_autotune(ss3d_context* ctx, const ss3d_spectrum* spec, int nx, int ny, 
 int ss3d_tuning_get(const ss3d_context* ctx, int kind, int logn);
 int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
 
+/* Context options (no counterpart in the reference).
+ * SS3D_OPT_PIPELINE (default 0): FP32 realizations run the fill pass of batch k+1 on a second
+ * stream while the remaining passes of batch k run, double-buffering the fill output. */
+#define SS3D_OPT_PIPELINE 1
+int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ EXE_PATH = os.path.join(PKG, "bin", "specsim3d")
 
 FP64 = 0
 FP32 = 1
+OPT_PIPELINE = 1
 
 ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device"}
 
@@ -73,6 +74,7 @@ SIGNATURES = {
     "ss3d_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ss3d_tuning_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ss3d_tuning_set": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ss3d_context_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
 }
 
@@ -205,6 +207,9 @@ class Engine:
         lx, ly, lz = (int(v).bit_length() - 1 for v in (npx, npy, npz))
         kinds = {"real_z_fp32": (6, lz), "real_y_fp32": (7, ly), "real_x_c2r_fp32": (8, max(lx - 1, 0))}
         return {k: lib().ss3d_tuning_get(self.handle, *v) for k, v in kinds.items()}
+
+    def set_option(self, option, value):
+        check(lib().ss3d_context_set_option(self.handle, option, value))
 
     def tuning_set(self, kind, logn, rm):
         check(lib().ss3d_tuning_set(self.handle, kind, logn, rm))

@@ -171,27 +171,31 @@ __device__ __forceinline__ Column column_of(const PassArgs& a, uint32_t rb, int 
     c.tpar = pm_mul(rb, __ldg(a.txp + kxp));
     return c;
 }
+// Row part of the classification (shared by every column): the point's row, the drawing row
+// of its site, and how the row compares with its partner row.
 struct Site {
     uint32_t row, site;
-    bool conj, selfc;
+    bool gt, eq;
 };
-__device__ __forceinline__ Site site_of(const PassArgs& a, const Column& c, int ky, int kz) {
+__device__ __forceinline__ Site site_of(const PassArgs& a, int ky, int kz) {
     const int kyp = (a.npy - ky) & (a.npy - 1), kzp = (a.npz - kz) & (a.npz - 1);
     const uint32_t row = ky + a.npy * kz, rowp = kyp + a.npy * kzp;
-    const bool gt = row > rowp, eq = row == rowp;
     Site s;
     s.row = row;
-    s.site = gt ? rowp : row;
-    s.conj = gt || (eq && c.xgt);
-    s.selfc = eq && c.xeq;
+    s.gt = row > rowp;
+    s.eq = row == rowp;
+    s.site = s.gt ? rowp : row;
     return s;
 }
+__device__ __forceinline__ bool site_conj(const Site& s, const Column& c) { return s.gt || (s.eq && c.xgt); }
+__device__ __forceinline__ bool site_self(const Site& s, const Column& c) { return s.eq && c.xeq; }
 
 // one coefficient (used where a single value is needed: C2R 1D fill, one-point x axis)
 __device__ __forceinline__ f2 fill_at(const PassArgs& a, const Column& c, int ky, int kz) {
-    const Site s = site_of(a, c, ky, kz);
-    const uint32_t st = pm_mul(__ldg(a.rowf + s.site), s.conj ? c.tpar : c.town);
-    return coef(__ldg(a.amp32h + c.kx + a.hx * s.row), st, a.igauss, s.conj, s.selfc);
+    const Site s = site_of(a, ky, kz);
+    const bool cj = site_conj(s, c);
+    const uint32_t st = pm_mul(__ldg(a.rowf + s.site), cj ? c.tpar : c.town);
+    return coef(__ldg(a.amp32h + c.kx + a.hxa * s.row), st, a.igauss, cj, site_self(s, c));
 }
 
 // ------------------------------------------------------------ reductions
@@ -236,69 +240,95 @@ __device__ __forceinline__ int out_pos(int b, int r) {
 }
 
 // ------------------------------------------------------------ strided pass (y or z axis)
-// Tile: P consecutive columns i0 (contiguous in HBM) x the whole axis (2^L).  Thread tid
-// handles column q = tid % P and butterfly slot g = tid / P.  Element offsets inside the tile
-// are 32-bit and relative to a wave-uniform base, strides are scalars: one VALU add per access.
+// Tile: P consecutive columns i0 (contiguous in HBM, P even) x the whole axis (2^L).  Thread
+// tid handles the column pair (i0base + 2q, +1), q = tid % (P/2), and butterfly slot
+// g = tid / (P/2): every HBM and LDS access moves 16 bytes (two complex values).  Element
+// offsets inside the tile are 32-bit and relative to a wave-uniform base; strides are scalars.
+// Buffers have an even row width, so the pair partner of the last active column is padding.
+//
+// Half-spectrum column 0 carries the Nyquist column kx = npx/2 in its imaginary part:
+// c'(0, ky, kz) = c(0, ky, kz) + i c(npx/2, ky, kz).  Both planes are Hermitian in (ky, kz),
+// so after the y and z transforms column 0 holds g0 + i gN with g0, gN real.
 template <int L, int P, int RM>
 struct StridedCfg {
-    static constexpr int N = 1 << L, EL = eptlog(L, RM), EPT = 1 << EL, G = N / EPT, NT = G * P;
+    static_assert(P % 2 == 0, "column pairs");
+    static constexpr int N = 1 << L, EL = eptlog(L, RM), EPT = 1 << EL, G = N / EPT, NT = G * (P / 2);
     static constexpr int M = npass(L, RM);
     static constexpr bool PADPOS = P < 16;
     static constexpr int ppos(int pos) { return PADPOS ? pos + (pos >> EL) : pos; }
     static constexpr size_t lds_elems() { return (size_t)(N + (PADPOS ? (N >> EL) : 0)) * P; }
 };
 
+__device__ __forceinline__ float4 pack2(f2 a, f2 b) { return make_float4(a.x, a.y, b.x, b.y); }
+
 template <int L, int P, int IN, int AXIS, int RM>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(PassArgs a) {
     using C = StridedCfg<L, P, RM>;
-    constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M;
+    constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, PH = P / 2;
     static_assert(M > 0, "strided passes have length >= 2");
-    f2* lds = reinterpret_cast<f2*>(fast_smem);
+    float4* lds4 = reinterpret_cast<float4*>(fast_smem);
     const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);
     const int tid = threadIdx.x;
-    const int q = tid % P, g = tid / P;
+    const int q = tid % PH, g = tid / PH;
     const int tile = blockIdx.x;
     const int i0base = (tile % a.ntiles0) * P;
     const int third = tile / a.ntiles0;
     const int b = blockIdx.y;
-    const int i0 = i0base + q;
-    const bool active = i0 < a.n0;
-    f2 v[EPT];
+    const int c0 = i0base + 2 * q;
+    const bool active = c0 < a.n0;
+    f2 v0[EPT], v1[EPT];
 
     // pass-0 operands: positions g + r*G
     if constexpr (IN == IN_LOAD) {
-        const f2* __restrict__ src = reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride +
-                                     (int64_t)third * a.in_sT + i0base;
-        const uint32_t sA = (uint32_t)a.in_sA;
+        const float4* __restrict__ src = reinterpret_cast<const float4*>(
+            reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride + (int64_t)third * a.in_sT + i0base);
+        const uint32_t sA = (uint32_t)a.in_sA >> 1;  // in float4 units
         const uint32_t o0 = q + (uint32_t)g * sA, step = (uint32_t)G * sA;
         if (active) {
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) v[r] = src[o0 + r * step];
+            for (int r = 0; r < EPT; ++r) {
+                const float4 w = src[o0 + r * step];
+                v0[r] = mk(w.x, w.y);
+                v1[r] = mk(w.z, w.w);
+            }
         } else {
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) v[r] = mk(0.f, 0.f);
+            for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
         }
-    } else {  // IN_FILL32H: half-spectrum coefficients; AXIS 2: (ky, kz) = (third, pos), AXIS 1: (pos, third)
+    } else {  // IN_FILL32H: AXIS 2: (ky, kz) = (third, pos), AXIS 1: (pos, third)
         if (active) {
-            const Column c = column_of(a, __ldg(a.rbase + b), i0);
+            const uint32_t rb = __ldg(a.rbase + b);
+            const Column ca = column_of(a, rb, c0), cb = column_of(a, rb, c0 + 1);
+            const bool pack = c0 == 0 && a.npx >= 2;  // column 0 also carries the Nyquist column
             uint32_t rowv[EPT];
-            float ampv[EPT];
+            float2 ampv[EPT];
             Site sv[EPT];
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
                 const int pos = g + r * G;
-                sv[r] = AXIS == 2 ? site_of(a, c, third, pos) : site_of(a, c, pos, third);
+                sv[r] = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
                 rowv[r] = __ldg(a.rowf + sv[r].site);
-                ampv[r] = __ldg(a.amp32h + i0 + a.hx * sv[r].row);
+                ampv[r] = __ldg(reinterpret_cast<const float2*>(a.amp32h + c0 + a.hxa * sv[r].row));
             }
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
-                const uint32_t st = pm_mul(rowv[r], sv[r].conj ? c.tpar : c.town);
-                v[r] = coef(ampv[r], st, a.igauss, sv[r].conj, sv[r].selfc);
+                // the site row is shared by the pair; x order decides only inside self rows
+                const bool j0 = site_conj(sv[r], ca), j1 = site_conj(sv[r], cb);
+                v0[r] = coef(ampv[r].x, pm_mul(rowv[r], j0 ? ca.tpar : ca.town), a.igauss, j0, site_self(sv[r], ca));
+                v1[r] = coef(ampv[r].y, pm_mul(rowv[r], j1 ? cb.tpar : cb.town), a.igauss, j1, site_self(sv[r], cb));
+            }
+            if (pack) {
+                const Column cn = column_of(a, rb, a.npx >> 1);
+#pragma unroll
+                for (int r = 0; r < EPT; ++r) {
+                    const int pos = g + r * G;
+                    const f2 zn = AXIS == 2 ? fill_at(a, cn, third, pos) : fill_at(a, cn, pos, third);
+                    v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
+                }
             }
         } else {
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) v[r] = mk(0.f, 0.f);
+            for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
         }
     }
 
@@ -314,24 +344,32 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
 #pragma unroll
                     for (int i = 0; i < NBp; ++i)
 #pragma unroll
-                        for (int r = 0; r < (1 << LRp); ++r)
-                            lds[C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r)) * P + q] = v[i * (1 << LRp) + r];
+                        for (int r = 0; r < (1 << LRp); ++r) {
+                            const int e = i * (1 << LRp) + r;
+                            lds4[C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r)) * PH + q] = pack2(v0[e], v1[e]);
+                        }
                     __syncthreads();
 #pragma unroll
                     for (int i = 0; i < NB; ++i)
 #pragma unroll
-                        for (int r = 0; r < (1 << LR); ++r)
-                            v[i * (1 << LR) + r] = lds[C::ppos(g + G * i + r * (N >> LR)) * P + q];
+                        for (int r = 0; r < (1 << LR); ++r) {
+                            const int e = i * (1 << LR) + r;
+                            const float4 w = lds4[C::ppos(g + G * i + r * (N >> LR)) * PH + q];
+                            v0[e] = mk(w.x, w.y);
+                            v1[e] = mk(w.z, w.w);
+                        }
                 }
-                radix_pass<L, Jc, G, RM>(v, g, tw);
+                radix_pass<L, Jc, G, RM>(v0, g, tw);
+                radix_pass<L, Jc, G, RM>(v1, g, tw);
             }(),
             ...);
     }(std::make_integer_sequence<int, M>{});
 
     // last pass outputs: butterfly b = g + G*i, element r at out_pos; positions >= n_out pruned
     if (active) {
-        f2* __restrict__ dst = reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base;
-        const uint32_t sA = (uint32_t)a.out_sA;
+        float4* __restrict__ dst = reinterpret_cast<float4*>(
+            reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base);
+        const uint32_t sA = (uint32_t)a.out_sA >> 1;
         constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
         const int nout = a.n_out;
 #pragma unroll
@@ -339,7 +377,8 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
 #pragma unroll
             for (int r = 0; r < (1 << LRl); ++r) {
                 const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
-                if (pos < nout) dst[q + (uint32_t)pos * sA] = v[i * (1 << LRl) + r];
+                const int e = i * (1 << LRl) + r;
+                if (pos < nout) dst[q + (uint32_t)pos * sA] = pack2(v0[e], v1[e]);
             }
     }
 }
@@ -376,11 +415,13 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
             const f2* __restrict__ src = reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride +
                                          (int64_t)blockIdx.x * P * a.in_row;
             const uint32_t base = (uint32_t)q * (uint32_t)a.in_row;
+            // column 0 = g0 + i gN (packed Nyquist column): X[0] = Re, X[N] = Im
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
                 const int k = g + r * G;
                 xa[r] = src[base + k];
-                xb[r] = src[base + (N - k)];
+                xb[r] = k == 0 ? mk(xa[r].y, 0.f) : src[base + (N - k)];
+                if (k == 0) xa[r].y = 0.f;
             }
         } else {  // IN_FILLH: the 1D grid (npy = npz = 1) filled in place
             const uint32_t rb = __ldg(a.rbase + b);
@@ -389,14 +430,17 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
                 const int k = g + r * G;
                 xa[r] = fill_at(a, column_of(a, rb, k), 0, 0);
                 xb[r] = fill_at(a, column_of(a, rb, N - k), 0, 0);
+                if (k == 0) {  // DC and Nyquist kept real
+                    xa[r].y = 0.f;
+                    xb[r].y = 0.f;
+                }
             }
         }
 #pragma unroll
         for (int r = 0; r < EPT; ++r) {
             const int k = g + r * G;
-            f2 A = xa[r];
-            if (k == 0) A.y = 0.f;                       // DC kept real
-            const f2 B = mk(xb[r].x, k == 0 ? 0.f : -xb[r].y);  // conj X[N-k]; Nyquist kept real
+            const f2 A = xa[r];
+            const f2 B = mk(xb[r].x, -xb[r].y);  // conj X[N-k]
             const f2 E = add(A, B);
             const f2 O = mul(__ldg(tw2 + k), sub(A, B));
             v[r] = mk(E.x - O.y, E.y + O.x);

@@ -7,7 +7,7 @@
 namespace ss3d {
 namespace fastlaunch {
 
-constexpr int strided_P(int L) { return L <= 7 ? 64 : (8192 >> L); }
+constexpr int strided_P(int L) { return L <= 7 ? 64 : (L >= 12 ? 2 : (8192 >> L)); }
 
 template <typename K>
 hipError_t launch_k(K k, size_t lds, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s, bool& attr) {

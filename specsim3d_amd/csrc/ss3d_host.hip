@@ -180,6 +180,12 @@ struct ss3d_context {
     // FP32 W_n tables for n = 2^0..2^kMaxLogN: entry m = exp(+2 pi i m/n) at offset (n - 1) + m
     DevBuf tw32n;
     DevBuf wa, wb, partial, stats, dout, rbase, diag;
+    // FP32 pipeline: the fill pass of batch k+1 (this->stream) overlaps the remaining passes of
+    // batch k (stream2); the fill output is double-buffered (pipe[0], pipe[1])
+    hipStream_t stream2 = nullptr;
+    DevBuf pipe[2], partial2;
+    hipEvent_t ev_fill[2] = {nullptr, nullptr}, ev_rest[2] = {nullptr, nullptr}, ev_join = nullptr;
+    bool pipeline = false;  // SS3D_OPT_PIPELINE
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
     // per-pass event timing (ss3d_context_profile)
@@ -194,6 +200,8 @@ struct ss3d_context {
     std::map<std::pair<int, int>, int> tune;
     std::vector<hipEvent_t> event_pool;
     ~ss3d_context() {
+        for (auto e : {ev_fill[0], ev_fill[1], ev_rest[0], ev_rest[1], ev_join})
+            if (e) (void)hipEventDestroy(e);
         for (auto& r : prof) {
             (void)hipEventDestroy(r.e0);
             (void)hipEventDestroy(r.e1);
@@ -209,7 +217,7 @@ struct RngTables {
 
 struct ss3d_spectrum {
     ss3d_context* ctx = nullptr;
-    int npx = 0, npy = 0, npz = 0, hx = 0;
+    int npx = 0, npy = 0, npz = 0, hx = 0, hxa = 0;
     DevBuf amp64, amp32h;
     RngTables rng[2];
     ss3d_spec_diag diag{};
@@ -352,8 +360,8 @@ double pass_bytes(const Pass& p) {
     const double ec = p.prec == 64 ? 16.0 : 8.0, er = ec / 2;
     const double n = double(int64_t(1) << a.logn);
     const double cols = a.axis == 0 ? double(a.nrows) : double(a.n0) * a.nthird;
-    if (p.fast == 2 || p.fast == 3)  // C2R x pass: half rows in, real cropped rows out
-        return (p.in == IN_LOADH || p.fast == 3 ? cols * a.hx * ec : a.hx * 4.0) + cols * a.n_out * er;
+    if (p.fast == 2 || p.fast == 3)  // C2R x pass: half rows (npx/2 complex) in, real cropped rows out
+        return (p.in == IN_LOADH || p.fast == 3 ? cols * std::max(1, a.npx / 2) * ec : a.hx * 4.0) + cols * a.n_out * er;
     double in = 0, out = 0;
     switch (p.in) {
         case IN_LOAD: in = cols * n * ec; break;
@@ -383,7 +391,7 @@ hipEvent_t take_event(ss3d_context* c) {
     return e;
 }
 
-int run_pass(ss3d_context* c, Pass& p, int nbatch) {
+int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& partial) {
     p.a.logp = p.sh.logp;
     int P = 1 << p.sh.logp;
     if (p.fast == 1) P = fast_strided_tile(p.a.logn);
@@ -398,29 +406,29 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch) {
     p.a.nblocks = p.nblocks;
     if (p.nblocks <= 0) return fail(SS3D_EINVAL, "empty pass");
     if (p.out == OUT_SPEC || p.out == OUT_REAL) {
-        SS3D_TRY(c->partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), c->stream));
-        p.a.partial = c->partial.as<double>();
+        SS3D_TRY(partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), strm));
+        p.a.partial = partial.as<double>();
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->profiling) {
         e0 = take_event(c);
         e1 = take_event(c);
         if (!e0 || !e1) return fail(SS3D_EHIP, "hipEventCreate failed");
-        HIP_TRY(hipEventRecord(e0, c->stream));
+        HIP_TRY(hipEventRecord(e0, strm));
     }
     hipError_t e;
     if (p.fast == 1)
-        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.nblocks, nbatch, p.a, c->stream);
+        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.nblocks, nbatch, p.a, strm);
     else if (p.fast == 2)
-        e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, c->stream);
+        e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, strm);
     else if (p.fast == 3)
-        e = launch_fast_x1(p.nblocks, nbatch, p.a, c->stream);
+        e = launch_fast_x1(p.nblocks, nbatch, p.a, strm);
     else
-        e = launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, c->stream);
+        e = launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, strm);
     if (e != hipSuccess) return fail(SS3D_EHIP, "pass launch (prec %d in %d out %d nt %d): %s", p.prec, p.in, p.out,
                                      p.sh.nt, hipGetErrorString(e));
     if (c->profiling) {
-        HIP_TRY(hipEventRecord(e1, c->stream));
+        HIP_TRY(hipEventRecord(e1, strm));
         c->prof.push_back({p.kind, e0, e1, pass_bytes(p) * nbatch});
     }
     return SS3D_OK;
@@ -433,6 +441,7 @@ PassArgs base_args(const ss3d_spectrum* sp) {
     a.npy = sp->npy;
     a.npz = sp->npz;
     a.hx = sp->hx;
+    a.hxa = sp->hxa;
     return a;
 }
 
@@ -465,6 +474,12 @@ int ss3d_context_create(int device, ss3d_context** out) {
     std::unique_ptr<ss3d_context> c(new ss3d_context);
     c->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_fill[i], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->ev_rest[i], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     SS3D_TRY(build_twiddles(c.get()));
     *out = c.release();
     return SS3D_OK;
@@ -474,15 +489,18 @@ int ss3d_context_destroy(ss3d_context* ctx) {
     if (!ctx) return SS3D_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    hipStream_t s = ctx->stream;
+    (void)hipStreamSynchronize(ctx->stream2);
+    hipStream_t s = ctx->stream, s2 = ctx->stream2;
     delete ctx;
     (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
     return SS3D_OK;
 }
 
 int ss3d_context_synchronize(ss3d_context* ctx) {
     if (!ctx) return fail(SS3D_EINVAL, "null context");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream2));
     return SS3D_OK;
 }
 
@@ -543,10 +561,12 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     sp->npy = npy;
     sp->npz = npz;
     sp->hx = npx >= 2 ? npx / 2 + 1 : 1;
+    sp->hxa = (sp->hx + 1) & ~1;
     const int64_t ntot = (int64_t)npx * npy * npz;
-    const int64_t nh = (int64_t)sp->hx * npy * npz;
+    const int64_t nh = (int64_t)sp->hxa * npy * npz;
     SS3D_TRY(sp->amp64.ensure(ntot * sizeof(double), c->stream));
     SS3D_TRY(sp->amp32h.ensure(nh * sizeof(float), c->stream));
+    HIP_TRY(hipMemsetAsync(sp->amp32h.p, 0, nh * sizeof(float), c->stream));  // padding column stays 0
     SS3D_TRY(c->wa.ensure(ntot * 2 * sizeof(double), c->stream));
     SS3D_TRY(c->diag.ensure(8 * sizeof(double), c->stream));
 
@@ -579,7 +599,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     px.a.out = c->wa.p;
     px.a.out_row = npx;
     px.a.tw = tw_ptr(c->tw64f, lx, 16);
-    SS3D_TRY(run_pass(c, px, 1));
+    SS3D_TRY(run_pass(c, px, 1, c->stream, c->partial));
     Pass* last = &px;
     Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a, 1};
     if (has_y) {
@@ -593,7 +613,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
         py.a.in_sA = py.a.out_sA = npx;
         py.a.in_sT = py.a.out_sT = (int64_t)npx * npy;
         py.a.tw = tw_ptr(c->tw64f, ly, 16);
-        SS3D_TRY(run_pass(c, py, 1));
+        SS3D_TRY(run_pass(c, py, 1, c->stream, c->partial));
         last = &py;
     }
     Pass pz{64, IN_LOAD, OUT_SPEC, strided_shape(lz, npx, 128, 16, c->target_e64, emax), 0, a, 2};
@@ -608,7 +628,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
         pz.a.in_sA = pz.a.out_sA = (int64_t)npx * npy;
         pz.a.in_sT = pz.a.out_sT = npx;
         pz.a.tw = tw_ptr(c->tw64f, lz, 16);
-        SS3D_TRY(run_pass(c, pz, 1));
+        SS3D_TRY(run_pass(c, pz, 1, c->stream, c->partial));
         last = &pz;
     }
     hipError_t e = launch_diag_finalize(c->partial.as<double>(), last->nblocks, c->diag.as<double>(), c->stream);
@@ -630,6 +650,7 @@ int ss3d_context_profile(ss3d_context* c, int enable) {
     if (!c) return fail(SS3D_EINVAL, "null context");
     if (enable) {
         HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream2));
         for (auto& r : c->prof) {
             c->event_pool.push_back(r.e0);
             c->event_pool.push_back(r.e1);
@@ -643,6 +664,7 @@ int ss3d_context_profile(ss3d_context* c, int enable) {
 int ss3d_context_profile_read(ss3d_context* c, double* ms, double* bytes, int64_t* launches, int nkinds) {
     if (!c || !ms || !bytes || !launches || nkinds < 1) return fail(SS3D_EINVAL, "null argument");
     HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream2));
     for (int k = 0; k < nkinds; ++k) {
         ms[k] = 0;
         bytes[k] = 0;
@@ -686,6 +708,8 @@ int ss3d_spectrum_amplitude(const ss3d_spectrum* sp, double* host_amp) {
 
 // ---------------------------------------------------------------- realizations
 namespace {
+
+int batch_size(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal);
 
 int tuned_rm(const ss3d_context* c, int kind, int L) {
     auto it = c->tune.find({kind, L});
@@ -741,7 +765,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p1.a.out_row = nx;
             p1.a.out_bstride = ncrop;
         }
-        SS3D_TRY(run_pass(c, p1, nb));
+        SS3D_TRY(run_pass(c, p1, nb, c->stream, c->partial));
         last = &p1;
         if (has_y) {
             p2 = Pass{64, IN_LOAD, has_z ? OUT_STORE : OUT_REAL, strided_shape(ly, nx, 128, 16, c->target_e64, emax), 0, a, 4};
@@ -759,7 +783,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p2.a.out_sT = (int64_t)nx * ny;
             p2.a.out_bstride = has_z ? b2 : ncrop;
             p2.a.tw = tw_ptr(c->tw64i, ly, 16);
-            SS3D_TRY(run_pass(c, p2, nb));
+            SS3D_TRY(run_pass(c, p2, nb, c->stream, c->partial));
             last = &p2;
         }
         if (has_z) {
@@ -778,77 +802,153 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p3.a.out_sT = nx;
             p3.a.out_bstride = ncrop;
             p3.a.tw = tw_ptr(c->tw64i, lz, 16);
-            SS3D_TRY(run_pass(c, p3, nb));
+            SS3D_TRY(run_pass(c, p3, nb, c->stream, c->partial));
             last = &p3;
         }
     } else {
-        // FP32: z -> y over the half spectrum kx <= npx/2, then the C2R x pass
-        if (lx > kMaxFastLog || ly > kMaxFastLog || lz > kMaxFastLog)
-            return fail(SS3D_ETOOBIG, "periodic axis longer than %d", 1 << kMaxFastLog);
-        const int64_t bz = (int64_t)hx * npy * nz, by = (int64_t)hx * ny * nz;
-        SS3D_TRY(c->wa.ensure((size_t)std::max<int64_t>(bz, 1) * nb * 8, c->stream));
-        SS3D_TRY(c->wb.ensure((size_t)std::max<int64_t>(by, 1) * nb * 8, c->stream));
-        const float2* tw32 = c->tw32n.as<float2>();
-        auto wtab = [&](int L) { return static_cast<const void*>(tw32 + ((size_t(1) << L) - 1)); };
-        const void* prev = nullptr;
-        int64_t prev_b = 0;
-        if (has_z) {
-            p1 = Pass{32, IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 6, 1};
-            p1.rm = tuned_rm(c, 6, lz);
-            p1.a.axis = 2;
-            p1.a.logn = lz;
-            p1.a.n0 = hx;
-            p1.a.nthird = npy;
-            p1.a.n_out = nz;
-            p1.a.out = c->wa.p;
-            p1.a.out_sA = (int64_t)hx * npy;
-            p1.a.out_sT = hx;
-            p1.a.out_bstride = bz;
-            p1.a.tw = wtab(lz);
-            SS3D_TRY(run_pass(c, p1, nb));
-            prev = c->wa.p;
-            prev_b = bz;
-        }
-        if (has_y) {
-            p2 = Pass{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 7, 1};
-            p2.rm = tuned_rm(c, 7, ly);
-            p2.a.axis = 1;
-            p2.a.logn = ly;
-            p2.a.n0 = hx;
-            p2.a.nthird = nz;
-            p2.a.n_out = ny;
-            p2.a.in = prev;
-            p2.a.in_sA = hx;
-            p2.a.in_sT = (int64_t)hx * npy;
-            p2.a.in_bstride = prev_b;
-            p2.a.out = c->wb.p;
-            p2.a.out_sA = hx;
-            p2.a.out_sT = (int64_t)hx * ny;
-            p2.a.out_bstride = by;
-            p2.a.tw = wtab(ly);
-            SS3D_TRY(run_pass(c, p2, nb));
-            prev = c->wb.p;
-            prev_b = by;
-        }
-        // x pass: npx >= 2 -> C2R on complex length npx/2; npx = 1 -> real part of each row
-        p3 = Pass{32, prev ? IN_LOADH : IN_FILLH, OUT_REAL, Shape{0, 0}, 0, a, 8, npx >= 2 ? 2 : 3};
-        p3.rm = tuned_rm(c, 8, npx >= 2 ? lx - 1 : 0);
-        p3.a.axis = 0;
-        p3.a.logn = npx >= 2 ? lx - 1 : 0;
-        p3.a.nrows = ny * nz;
-        p3.a.n_out = nx;
-        p3.a.in = prev;
-        p3.a.in_row = hx;
-        p3.a.in_bstride = prev_b;
-        p3.a.out = dout;
-        p3.a.out_row = nx;
-        p3.a.out_bstride = ncrop;
-        p3.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
-        SS3D_TRY(run_pass(c, p3, nb));
-        last = &p3;
+        return fail(SS3D_EINVAL, "enqueue_batch handles the FP64 path only");
     }
     hipError_t e = launch_stats_finalize(c->partial.as<double>(), last->nblocks, nb, (double)ncrop, dstats, c->stream);
     if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
+    return SS3D_OK;
+}
+
+// Grows a buffer shared by both streams (waits for both before freeing the old one).
+int ensure2(ss3d_context* c, DevBuf& b, size_t bytes) {
+    if (bytes <= b.cap) return SS3D_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream2));
+    return b.ensure(bytes, c->stream);
+}
+
+// FP32 realizations r0..r0+nreal-1: z -> y passes over the half spectrum (column 0 also carries
+// the Nyquist column), then the C2R x pass, in batches of nb realizations.  The fill pass (the
+// first one) runs on c->stream and writes one of two pipe buffers; the other passes and the
+// statistics run on c->stream2, so the fill of batch k+1 (arithmetic-bound) overlaps the
+// memory-bound passes of batch k.
+int enqueue_fp32(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nreal, int nx, int ny,
+                 int nz, void* dout, double* dstats) {
+    const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
+    const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
+    if (lx > kMaxFastLog || ly > kMaxFastLog || lz > kMaxFastLog)
+        return fail(SS3D_ETOOBIG, "periodic axis longer than %d", 1 << kMaxFastLog);
+    SS3D_TRY(build_rng(sp, igauss));
+    SS3D_TRY(ensure2(c, c->rbase, (size_t)nreal * 4));
+    {
+        const uint64_t dmod = (uint64_t)draws_per_realization(npx, npy, npz, igauss) % (kPmM - 1);
+        hipLaunchKernelGGL(rbase_kernel, dim3((nreal + 63) / 64), dim3(64), 0, c->stream, seed_state(iseed), dmod,
+                           (uint64_t)r0, nreal, c->rbase.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+    }
+    PassArgs a = base_args(sp);
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.igauss = igauss;
+    a.amp32h = sp->amp32h.as<float>();
+    a.rowf = sp->rng[igauss].rowf.as<uint32_t>();
+    a.txp = sp->rng[igauss].txp.as<uint32_t>();
+    const bool has_y = npy > 1, has_z = npz > 1;
+    const int64_t ncrop = (int64_t)nx * ny * nz;
+    const int nb = batch_size(sp, nx, ny, nz, SS3D_FP32, nreal);
+    const int hc = npx >= 2 ? npx / 2 : 1, hcs = std::max(2, hc);
+    const int64_t bz = (int64_t)hcs * npy * nz, by = (int64_t)hcs * ny * nz;
+    const float2* tw32 = c->tw32n.as<float2>();
+    auto wtab = [&](int L) { return static_cast<const void*>(tw32 + ((size_t(1) << L) - 1)); };
+
+    // pass templates; `in`/`out` buffers and rbase are set per batch
+    Pass pz{32, IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 6, 1};
+    pz.rm = tuned_rm(c, 6, lz);
+    pz.a.axis = 2;
+    pz.a.logn = lz;
+    pz.a.n0 = hc;
+    pz.a.nthird = npy;
+    pz.a.n_out = nz;
+    pz.a.out_sA = (int64_t)hcs * npy;
+    pz.a.out_sT = hcs;
+    pz.a.out_bstride = bz;
+    pz.a.tw = wtab(lz);
+    Pass py{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 7, 1};
+    py.rm = tuned_rm(c, 7, ly);
+    py.a.axis = 1;
+    py.a.logn = ly;
+    py.a.n0 = hc;
+    py.a.nthird = nz;
+    py.a.n_out = ny;
+    py.a.in_sA = hcs;
+    py.a.in_sT = (int64_t)hcs * npy;
+    py.a.in_bstride = bz;
+    py.a.out_sA = hcs;
+    py.a.out_sT = (int64_t)hcs * ny;
+    py.a.out_bstride = by;
+    py.a.tw = wtab(ly);
+    // x pass: npx >= 2 -> C2R on complex length npx/2; npx = 1 -> real part of each row
+    Pass px{32, (has_y || has_z) ? IN_LOADH : IN_FILLH, OUT_REAL, Shape{0, 0}, 0, a, 8, npx >= 2 ? 2 : 3};
+    px.rm = tuned_rm(c, 8, npx >= 2 ? lx - 1 : 0);
+    px.a.axis = 0;
+    px.a.logn = npx >= 2 ? lx - 1 : 0;
+    px.a.nrows = ny * nz;
+    px.a.n_out = nx;
+    px.a.in_row = hcs;
+    px.a.in_bstride = has_y ? by : bz;
+    px.a.out_row = nx;
+    px.a.out_bstride = ncrop;
+    px.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
+
+    const bool staged = has_y || has_z;  // a fill pass followed by further passes
+    const int64_t pipe_elems = has_z ? bz : by;  // output of the fill pass
+    hipStream_t s2 = c->pipeline ? c->stream2 : c->stream;  // stream of the passes after the fill
+    const bool pipelined = staged && c->pipeline;
+    if (staged) {
+        for (int i = 0; i < 2; ++i) SS3D_TRY(ensure2(c, c->pipe[i], (size_t)std::max<int64_t>(pipe_elems, 1) * nb * 8));
+        if (has_z && has_y) SS3D_TRY(ensure2(c, c->wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
+    }
+    if (pipelined) {
+        HIP_TRY(hipEventRecord(c->ev_join, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_join, 0));
+    }
+    const int nbat = (nreal + nb - 1) / nb;
+    for (int k = 0; k < nbat; ++k) {
+        const int done = k * nb, cnt = std::min(nb, nreal - done);
+        const int buf = k & 1;
+        void* out = static_cast<char*>(dout) + (size_t)done * ncrop * 4;
+        double* st = dstats + 3 * (size_t)done;
+        const uint32_t* rb = c->rbase.as<uint32_t>() + done;
+        if (!staged) {
+            px.a.rbase = rb;
+            px.a.out = out;
+            SS3D_TRY(run_pass(c, px, cnt, c->stream, c->partial));
+            hipError_t e = launch_stats_finalize(c->partial.as<double>(), px.nblocks, cnt, (double)ncrop, st, c->stream);
+            if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
+            continue;
+        }
+        if (pipelined && k >= 2) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_rest[buf], 0));
+        Pass& fill = has_z ? pz : py;
+        fill.a.rbase = rb;
+        fill.a.out = c->pipe[buf].p;
+        SS3D_TRY(run_pass(c, fill, cnt, c->stream, c->partial));
+        if (pipelined) {
+            HIP_TRY(hipEventRecord(c->ev_fill[buf], c->stream));
+            HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fill[buf], 0));
+        }
+        DevBuf& part = pipelined ? c->partial2 : c->partial;
+        if (has_z && has_y) {
+            py.a.in = c->pipe[buf].p;
+            py.a.out = c->wb.p;
+            SS3D_TRY(run_pass(c, py, cnt, s2, part));
+            px.a.in = c->wb.p;
+        } else {
+            px.a.in = c->pipe[buf].p;
+        }
+        px.a.out = out;
+        SS3D_TRY(run_pass(c, px, cnt, s2, part));
+        hipError_t e = launch_stats_finalize(part.as<double>(), px.nblocks, cnt, (double)ncrop, st, s2);
+        if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
+        if (pipelined) HIP_TRY(hipEventRecord(c->ev_rest[buf], c->stream2));
+    }
+    if (pipelined) {
+        HIP_TRY(hipEventRecord(c->ev_join, c->stream2));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    }
     return SS3D_OK;
 }
 
@@ -869,11 +969,12 @@ int check_realize(ss3d_context* c, const ss3d_spectrum* sp, int igauss, int64_t 
 int batch_size(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal) {
     const int64_t per = prec == SS3D_FP64
                             ? ((int64_t)nx * sp->npy * sp->npz + (int64_t)nx * ny * sp->npz) * 16
-                            : ((int64_t)sp->hx * sp->npy * nz + (int64_t)sp->hx * ny * nz) * 8;
+                            : ((int64_t)std::max(2, sp->npx / 2) * (sp->npy * (int64_t)nz + (int64_t)ny * nz)) * 8;
     const int64_t budget = int64_t(4) << 30;
     int64_t nb = std::max<int64_t>(1, budget / std::max<int64_t>(per, 1));
-    const int64_t fill = std::max<int64_t>(1, (int64_t(1) << 24) / std::max<int64_t>((int64_t)sp->npx * sp->npy * sp->npz, 1));
-    nb = std::min<int64_t>(nb, std::max<int64_t>(fill, 1));
+    // small grids: enough realizations per launch to fill the GPU; large grids: up to 8
+    const int64_t fill = (int64_t(1) << 24) / std::max<int64_t>((int64_t)sp->npx * sp->npy * sp->npz, 1);
+    nb = std::min<int64_t>(nb, std::max<int64_t>(fill, 8));
     nb = std::min<int64_t>(nb, nreal);
     return (int)std::max<int64_t>(nb, 1);
 }
@@ -889,14 +990,18 @@ int ss3d_realize_device(ss3d_context* c, const ss3d_spectrum* spc, int iseed, in
     if (!dev_out) return fail(SS3D_EINVAL, "null device output");
     HIP_TRY(hipSetDevice(c->device));
     ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
-    const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
-    const size_t esz = prec == SS3D_FP64 ? 8 : 4;
     const int64_t ncrop = (int64_t)nx * ny * nz;
-    if (!dev_stats) SS3D_TRY(c->stats.ensure((size_t)nb * 3 * sizeof(double), c->stream));
+    if (prec == SS3D_FP32) {
+        if (!dev_stats) SS3D_TRY(ensure2(c, c->stats, (size_t)nreal * 3 * sizeof(double)));
+        return enqueue_fp32(c, sp, iseed, igauss, r0, nreal, nx, ny, nz, dev_out,
+                            dev_stats ? dev_stats : c->stats.as<double>());
+    }
+    const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
+    if (!dev_stats) SS3D_TRY(ensure2(c, c->stats, (size_t)nreal * 3 * sizeof(double)));
     for (int done = 0; done < nreal; done += nb) {
         const int cnt = std::min(nb, nreal - done);
-        void* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * esz;
-        double* st = dev_stats ? dev_stats + 3 * (size_t)done : c->stats.as<double>();
+        void* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * 8;
+        double* st = (dev_stats ? dev_stats : c->stats.as<double>()) + 3 * (size_t)done;
         SS3D_TRY(enqueue_batch(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, out, st));
     }
     return SS3D_OK;
@@ -909,15 +1014,19 @@ int ss3d_realize(ss3d_context* c, const ss3d_spectrum* spc, int iseed, int igaus
     if (!host_out) return fail(SS3D_EINVAL, "null host output");
     HIP_TRY(hipSetDevice(c->device));
     ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
-    const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
     const size_t esz = prec == SS3D_FP64 ? 8 : 4;
     const int64_t ncrop = (int64_t)nx * ny * nz;
-    SS3D_TRY(c->dout.ensure((size_t)nb * ncrop * esz, c->stream));
-    SS3D_TRY(c->stats.ensure((size_t)nb * 3 * sizeof(double), c->stream));
-    std::vector<double> hs((size_t)nb * 3);
-    for (int done = 0; done < nreal; done += nb) {
-        const int cnt = std::min(nb, nreal - done);
-        SS3D_TRY(enqueue_batch(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, c->dout.p, c->stats.as<double>()));
+    // realizations per device round trip: at least one batch, up to ~1 GiB of output
+    const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
+    const int64_t by_mem = std::max<int64_t>(1, (int64_t(1) << 30) / std::max<int64_t>(ncrop * (int64_t)esz, 1));
+    const int ch = (int)std::min<int64_t>(nreal, std::max<int64_t>(nb, by_mem));
+    SS3D_TRY(ensure2(c, c->dout, (size_t)ch * ncrop * esz));
+    SS3D_TRY(ensure2(c, c->stats, (size_t)ch * 3 * sizeof(double)));
+    std::vector<double> hs((size_t)ch * 3);
+    for (int done = 0; done < nreal; done += ch) {
+        const int cnt = std::min(ch, nreal - done);
+        SS3D_TRY(ss3d_realize_device(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, c->dout.p,
+                                     c->stats.as<double>()));
         HIP_TRY(hipMemcpyAsync(static_cast<char*>(host_out) + (size_t)done * ncrop * esz, c->dout.p,
                                (size_t)cnt * ncrop * esz, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipMemcpyAsync(hs.data(), c->stats.p, (size_t)cnt * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -943,17 +1052,17 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
     const int Lk[3] = {lz, ly, sp->npx >= 2 ? lx - 1 : 0};
     const int nb = batch_size(sp, nx, ny, nz, SS3D_FP32, 64);
     const int64_t ncrop = (int64_t)nx * ny * nz;
-    SS3D_TRY(c->dout.ensure((size_t)nb * ncrop * 4, c->stream));
-    SS3D_TRY(c->stats.ensure((size_t)nb * 3 * sizeof(double), c->stream));
+    SS3D_TRY(ensure2(c, c->dout, (size_t)nb * ncrop * 4));
+    SS3D_TRY(ensure2(c, c->stats, (size_t)nb * 3 * sizeof(double)));
     const bool was = c->profiling;
     double best[3] = {1e30, 1e30, 1e30};
     int pick[3] = {5, 5, 5};
     for (int rm = 3; rm <= 5; ++rm) {
         for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = rm;
-        SS3D_TRY(enqueue_batch(c, sp, 1, igauss, 0, nb, nx, ny, nz, SS3D_FP32, c->dout.p, c->stats.as<double>()));
+        SS3D_TRY(enqueue_fp32(c, sp, 1, igauss, 0, nb, nx, ny, nz, c->dout.p, c->stats.as<double>()));
         SS3D_TRY(ss3d_context_profile(c, 1));
         for (int r = 0; r < std::max(1, reps); ++r)
-            SS3D_TRY(enqueue_batch(c, sp, 1, igauss, 0, nb, nx, ny, nz, SS3D_FP32, c->dout.p, c->stats.as<double>()));
+            SS3D_TRY(enqueue_fp32(c, sp, 1, igauss, 0, nb, nx, ny, nz, c->dout.p, c->stats.as<double>()));
         double ms[SS3D_NPASS_KINDS], by[SS3D_NPASS_KINDS];
         int64_t nl[SS3D_NPASS_KINDS];
         SS3D_TRY(ss3d_context_profile_read(c, ms, by, nl, SS3D_NPASS_KINDS));
@@ -966,6 +1075,17 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
     SS3D_TRY(ss3d_context_profile(c, was ? 1 : 0));
     for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = pick[k];
     return SS3D_OK;
+}
+
+int ss3d_context_set_option(ss3d_context* c, int option, int value) {
+    if (!c) return fail(SS3D_EINVAL, "null context");
+    if (option == SS3D_OPT_PIPELINE) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream2));
+        c->pipeline = value != 0;
+        return SS3D_OK;
+    }
+    return fail(SS3D_EINVAL, "unknown option %d", option);
 }
 
 int ss3d_tuning_get(const ss3d_context* c, int kind, int logn) {

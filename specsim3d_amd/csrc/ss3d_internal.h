@@ -44,6 +44,7 @@ struct PassArgs {
     const void* tw;  // twiddle table of length 2^(logn-1) for this axis and direction
     // periodic grid and crop
     int npx, npy, npz, hx;  // hx = npx/2+1 (npx >= 2) or 1: half-spectrum width
+    int hxa;                // row stride of the FP32 amplitude: hx rounded up to even
     int nx, ny, nz;
     // covariance model (spectrum passes)
     int nst;
@@ -51,7 +52,7 @@ struct PassArgs {
     StructDev st[kMaxStruct];
     // coefficient fill (realization passes)
     const double* amp64;     // npx*npy*npz
-    const float* amp32h;     // hx*npy*npz
+    const float* amp32h;     // hxa*npy*npz (columns kx <= npx/2 used)
     const uint32_t* rowf;    // per drawing row: a^(g*(draw-point index of its first point))
     const uint32_t* txp;     // a^(g*i), i < npx
     const uint32_t* rbase;   // per realization: state before its first draw, times a

@@ -453,7 +453,7 @@ __global__ __launch_bounds__(NT) void pass_kernel(PassArgs a) {
             }
             const int64_t row = j + (int64_t)a.npy * k;
             if (a.amp64_out) a.amp64_out[i + (int64_t)a.npx * row] = amp;
-            if (a.amp32h_out && i < a.hx) a.amp32h_out[i + (int64_t)a.hx * row] = (float)amp;
+            if (a.amp32h_out && i < a.hx) a.amp32h_out[i + (int64_t)a.hxa * row] = (float)amp;
         }
         const int ops[4] = {1, 1, 0, 0};
         block_reduce_write<NT, 4>(acc, ops, a.partial + 4 * ((int64_t)b * a.nblocks + blockIdx.x));

@@ -168,3 +168,33 @@ def test_autotune_returns_a_plan(eng):
     assert set(plan.values()) <= {3, 4, 5}
     o = O.simulate(sts, 0.0, 1, 1, 1, 64, 64, 16, 1, 69069, 1, 0)
     assert rel_rms(a.astype(np.float64), o["res"]) < 1e-5
+
+
+@pytest.mark.parametrize("grid", [(3000, 5, 1), (5, 6000, 1), (4, 3, 2500), (700, 600, 1)])
+def test_long_axes(eng, grid):
+    """Axes up to 8192 points (the longest the passes take): FP32 vs FP64 of the same draws."""
+    say(f"[long] grid {grid}")
+    sts = [(2, 1.0, 30, 20, 10, 20, 0, 0)]
+    npg = S.periodic_grid(sts, 1, 1, 1, *grid, 0)
+    sp = eng.spectrum(sts, 0.1, 1, 1, 1, *npg)
+    f64, _, v64, _ = eng.realize(sp, 99, 1, 0, 1, *grid, S.FP64)
+    f32, _, v32, _ = eng.realize(sp, 99, 1, 0, 1, *grid, S.FP32)
+    sp.close()
+    assert rel_rms(f32.astype(np.float64), f64) < 1e-5
+    assert abs(v32[0] - v64[0]) < 1e-5 * max(1.0, v64[0])
+
+
+def test_stream_pipeline_gives_identical_results(eng):
+    """The two-stream FP32 pipeline (fill of batch k+1 beside the passes of batch k) changes no bit."""
+    sts = [(2, 0.7, 20, 10, 5, 30, 15, 0)]
+    grid = (100, 90, 40)
+    npg = S.periodic_grid(sts, 1, 1, 1, *grid, 1)
+    sp = eng.spectrum(sts, 0.3, 1, 1, 1, *npg)
+    a, ma, va, _ = eng.realize(sp, 77, 1, 0, 40, *grid, S.FP32)
+    eng.set_option(S.OPT_PIPELINE, 1)
+    try:
+        b, mb, vb, _ = eng.realize(sp, 77, 1, 0, 40, *grid, S.FP32)
+    finally:
+        eng.set_option(S.OPT_PIPELINE, 0)
+    sp.close()
+    assert np.array_equal(a, b) and np.array_equal(ma, mb) and np.array_equal(va, vb)
