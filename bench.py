@@ -89,6 +89,58 @@ def cpu_baseline(threads):
             "sample": f"oracle C restatement, 128^3 grid / 256^3 periodic, 1 realization incl. spectrum: {dt:.2f} s"}
 
 
+def serve(S, eng, rate, duration, seed, nreal=64, nslots=4, spot_frac=0.05, repeat_frac=0.3):
+    """Poisson arrivals of config-2-like requests (256^3, igauss=1, FP32 output left in HBM): 30% repeat
+    an earlier model (spectral-density cache), the rest draw ranges and angles anew; 5% carry an FP64 spot
+    check of realization 0.  Returns latency percentiles and throughput."""
+    import threading
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rng.poisson(rate * duration)))
+    arrivals = np.sort(rng.uniform(0.0, duration, n))
+    models = []
+    reqs = []
+    for i in range(n):
+        if models and rng.random() < repeat_frac:
+            m = models[int(rng.integers(len(models)))]
+        else:
+            m = [(2, 0.7, float(rng.uniform(60, 100)), float(rng.uniform(30, 50)), float(rng.uniform(5, 15)),
+                  float(rng.uniform(0, 180)), float(rng.uniform(-30, 30)), 0.0)]
+            models.append(m)
+        reqs.append(S.Server.request(i, m, NUGGET, 1, 1, 1, *GRID, 1, int(rng.integers(1, 2**31 - 1)), nreal, IGAUSS,
+                                     S.FP32, spot_check=bool(rng.random() < spot_frac)))
+    done = {}
+    srv = S.Server(eng, nslots=nslots)
+    try:
+        def collect():
+            while len(done) < n:
+                for c in srv.poll(timeout=0.2):
+                    done[c["id"]] = c
+        th = threading.Thread(target=collect)
+        th.start()
+        t_start = srv.clock()
+        for t, r in zip(arrivals, reqs):
+            while srv.clock() - t_start < t:
+                time.sleep(min(0.002, max(0.0, t - (srv.clock() - t_start))))
+            srv.submit(r)
+        th.join()
+    finally:
+        srv.close()
+    lat = np.array([c["t_done"] - c["t_submit"] for c in done.values()]) * 1e3
+    ok = all(c["status"] == 0 for c in done.values())
+    t_first = min(c["t_submit"] for c in done.values())
+    t_last = max(c["t_done"] for c in done.values())
+    cells = n * nreal * GRID[0] * GRID[1] * GRID[2]
+    spots = [c["spot_rel_rms"] for c in done.values() if c["spot_rel_rms"] >= 0]
+    return {"offered_req_s": rate, "seconds_of_arrivals": duration, "requests": n, "all_ok": ok,
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "throughput_gcells_s": round(cells / (t_last - t_first) / 1e9, 2),
+            "completed_req_s": round(n / (t_last - t_first), 2),
+            "cache_hit_frac": round(sum(c["cache_hit"] for c in done.values()) / n, 3),
+            "spot_checks": len(spots), "spot_max_rel_rms": max(spots) if spots else None}
+
+
 def load_traffic(kind):
     path = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if not os.path.exists(path):
@@ -107,6 +159,8 @@ def main():
     ap.add_argument("--nreal", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -163,6 +217,12 @@ def main():
     hip.free(out64)
     hip.free(stats)
     spec.close()
+
+    serving = None
+    if not args.no_serve:
+        sustained = serve(S, eng, args.serve_rate, 6.0, 1)
+        overload = serve(S, eng, 200.0, 2.0, 2)
+        serving = {"sustained": sustained, "overload": overload}
     eng.close()
 
     cells32 = nreal * ncell * args.steps
@@ -197,7 +257,8 @@ def main():
         "fp32_over_fp64": round(value / fp64_value, 2),
         "spectrum_ms": round(spec_ms, 2),
         "autotune": {"largest_radix_log2": plan, "seconds": round(tune_s, 2)},
-        "latency_ms": {"p50": None, "p99": None, "note": "serving loop not measured in this run"},
+        "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
+                        "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kind),
                      "algorithmic_bytes_per_launch": byt / nl, "ms_per_launch": ms / nl},

@@ -133,6 +133,45 @@ int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
 #define SS3D_OPT_PIPELINE 1
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
+/* ---------------------------------------------------------------- request loop
+ * No counterpart in the reference (one process per parameter file).  A server owns `nslots`
+ * lanes of a context and a worker thread.  Requests are admitted in arrival order as soon as a
+ * lane is free and their device footprint fits `hbm_budget` bytes (<= 0: 80% of free memory);
+ * requests admitted while others run overlap on the device.  Spectral densities are cached by
+ * (model, spacing, periodic grid), shared, and evicted least recently used when unreferenced.
+ * While a server runs, its context must not be used from other threads. */
+typedef struct ss3d_server ss3d_server;
+
+typedef struct ss3d_request {
+    int64_t id;
+    int32_t nst, iseed;
+    ss3d_structure st[10];
+    double nugget, dx, dy, dz;
+    int32_t nx, ny, nz, ipad;
+    int32_t igauss, nreal, precision;
+    int32_t spot_check; /* FP32 requests: also compute realization 0 in FP64 and report the RMS gap */
+    void* host_out;     /* optional: nreal*nx*ny*nz values copied here before completion */
+} ss3d_request;
+
+typedef struct ss3d_completion {
+    int64_t id;
+    int32_t status;    /* SS3D_OK or the error code */
+    int32_t cache_hit; /* spectral density found in the cache */
+    double t_submit, t_admit, t_done; /* seconds on the server clock */
+    double spot_rel_rms;              /* relative RMS FP32 vs FP64 of realization 0, or -1 */
+    double mean0, var0;               /* statistics of realization 0 */
+    int64_t out_bytes;
+} ss3d_completion;
+
+int ss3d_server_create(ss3d_context* ctx, int nslots, int64_t hbm_budget, ss3d_server** out);
+int ss3d_server_submit(ss3d_server* s, const ss3d_request* r);
+/* Returns up to `max` finished requests; waits up to timeout_s seconds when none is ready. */
+int ss3d_server_poll(ss3d_server* s, ss3d_completion* out, int max, int* n, double timeout_s);
+int ss3d_server_info(ss3d_server* s, int64_t* queued, int64_t* running, int64_t* cached, int64_t* cache_bytes);
+double ss3d_server_clock(const ss3d_server* s);
+/* Waits for every submitted request, then releases the lanes and the cache. */
+int ss3d_server_destroy(ss3d_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,23 @@ class Structure(ctypes.Structure):
                 ("az", ctypes.c_double), ("dip", ctypes.c_double), ("plg", ctypes.c_double)]
 
 
+class Request(ctypes.Structure):
+    """ss3d_request: one simulation request of the request loop."""
+    _fields_ = [("id", ctypes.c_int64), ("nst", ctypes.c_int32), ("iseed", ctypes.c_int32),
+                ("st", Structure * 10), ("nugget", ctypes.c_double), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                ("nz", ctypes.c_int32), ("ipad", ctypes.c_int32), ("igauss", ctypes.c_int32),
+                ("nreal", ctypes.c_int32), ("precision", ctypes.c_int32), ("spot_check", ctypes.c_int32),
+                ("host_out", ctypes.c_void_p)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int32), ("cache_hit", ctypes.c_int32),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("spot_rel_rms", ctypes.c_double), ("mean0", ctypes.c_double), ("var0", ctypes.c_double),
+                ("out_bytes", ctypes.c_int64)]
+
+
 class SpecDiag(ctypes.Structure):
     _fields_ = [("max_imag", ctypes.c_double), ("max_real", ctypes.c_double), ("c0", ctypes.c_double),
                 ("spos", ctypes.c_double), ("sneg", ctypes.c_double)]
@@ -75,6 +92,13 @@ SIGNATURES = {
     "ss3d_tuning_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ss3d_tuning_set": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ss3d_context_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "ss3d_server_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "ss3d_server_submit": (ctypes.c_int, [_vp, ctypes.c_void_p]),
+    "ss3d_server_poll": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.c_double]),
+    "ss3d_server_info": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int64)] * 4),
+    "ss3d_server_clock": (ctypes.c_double, [_vp]),
+    "ss3d_server_destroy": (ctypes.c_int, [_vp]),
     "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
 }
 
@@ -248,3 +272,66 @@ def simulate(engine, sts, nugget, dx, dy, dz, nx, ny, nz, ipad, iseed, nreal, ig
         diag = spec.diag
         spec.close()
     return dict(res=res, mean=mean, var=var, maximag=mi, diag=diag, np=grid)
+
+
+class Server:
+    """Request loop over an Engine: continuous batching of simulation requests on `nslots` lanes,
+    admission by HBM footprint, spectral-density cache (ss3d_server_* in include/specsim3d.h)."""
+
+    def __init__(self, engine, nslots=4, hbm_budget=0):
+        self.engine = engine
+        self.handle = _vp()
+        check(lib().ss3d_server_create(engine.handle, nslots, hbm_budget, ctypes.byref(self.handle)))
+        self._keep = {}
+
+    @staticmethod
+    def request(rid, sts, nugget, dx, dy, dz, nx, ny, nz, ipad, iseed, nreal, igauss, precision=FP32,
+                spot_check=False, host_out=None):
+        r = Request()
+        r.id = rid
+        r.nst = len(sts)
+        for i, t in enumerate(sts):
+            r.st[i].itype = int(t[0])
+            (r.st[i].sill, r.st[i].ahmax, r.st[i].ahmin, r.st[i].avert, r.st[i].az, r.st[i].dip,
+             r.st[i].plg) = map(float, t[1:8])
+        r.iseed, r.nugget, r.dx, r.dy, r.dz = iseed, nugget, dx, dy, dz
+        r.nx, r.ny, r.nz, r.ipad, r.igauss, r.nreal = nx, ny, nz, ipad, igauss, nreal
+        r.precision, r.spot_check = precision, 1 if spot_check else 0
+        r.host_out = host_out.ctypes.data if host_out is not None else None
+        return r
+
+    def submit(self, req, keep=None):
+        """Queue a request; `keep` (e.g. its host output array) stays referenced until completion."""
+        if keep is not None:
+            self._keep[req.id] = keep
+        check(lib().ss3d_server_submit(self.handle, ctypes.byref(req)))
+
+    def poll(self, max_n=64, timeout=0.1):
+        buf = (Completion * max_n)()
+        n = ctypes.c_int()
+        check(lib().ss3d_server_poll(self.handle, buf, max_n, ctypes.byref(n), timeout))
+        out = []
+        for i in range(n.value):
+            c = buf[i]
+            self._keep.pop(c.id, None)
+            out.append({f: getattr(c, f) for f, _ in Completion._fields_})
+        return out
+
+    def info(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        check(lib().ss3d_server_info(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(queued=v[0].value, running=v[1].value, cached=v[2].value, cache_bytes=v[3].value)
+
+    def clock(self):
+        return lib().ss3d_server_clock(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().ss3d_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

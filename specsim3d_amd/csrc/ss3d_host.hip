@@ -7,13 +7,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/specsim3d.h"
@@ -154,6 +159,41 @@ struct DevBuf {
     }
 };
 
+// One ordered lane of device work: a stream (plus a second one for the FP32 pipeline) and the
+// work buffers a spectrum build or a realization call needs.  A context has one lane; a server
+// has one per in-flight request, so their work overlaps on the device.
+struct Lane {
+    hipStream_t stream = nullptr, stream2 = nullptr;
+    DevBuf wa, wb, pipe[2], partial, partial2, rbase, stats, dout, diag;
+    hipEvent_t ev_fill[2] = {nullptr, nullptr}, ev_rest[2] = {nullptr, nullptr}, ev_join = nullptr;
+    int init() {
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+        for (int i = 0; i < 2; ++i) {
+            HIP_TRY(hipEventCreateWithFlags(&ev_fill[i], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ev_rest[i], hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        return SS3D_OK;
+    }
+    void sync() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (stream2) (void)hipStreamSynchronize(stream2);
+    }
+    ~Lane() {
+        sync();
+        for (auto e : {ev_fill[0], ev_fill[1], ev_rest[0], ev_rest[1], ev_join})
+            if (e) (void)hipEventDestroy(e);
+        // buffers are freed by their destructors after the streams went idle
+    }
+    void destroy_streams() {
+        sync();
+        if (stream) (void)hipStreamDestroy(stream);
+        if (stream2) (void)hipStreamDestroy(stream2);
+        stream = stream2 = nullptr;
+    }
+};
+
 __global__ void rbase_kernel(uint32_t s10, uint64_t dmod, uint64_t r0, int n, uint32_t* out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -179,13 +219,8 @@ struct ss3d_context {
     DevBuf tw64f, tw64i, tw32i;
     // FP32 W_n tables for n = 2^0..2^kMaxLogN: entry m = exp(+2 pi i m/n) at offset (n - 1) + m
     DevBuf tw32n;
-    DevBuf wa, wb, partial, stats, dout, rbase, diag;
-    // FP32 pipeline: the fill pass of batch k+1 (this->stream) overlaps the remaining passes of
-    // batch k (stream2); the fill output is double-buffered (pipe[0], pipe[1])
-    hipStream_t stream2 = nullptr;
-    DevBuf pipe[2], partial2;
-    hipEvent_t ev_fill[2] = {nullptr, nullptr}, ev_rest[2] = {nullptr, nullptr}, ev_join = nullptr;
-    bool pipeline = false;  // SS3D_OPT_PIPELINE
+    Lane main;               // the context's own lane (main.stream is `stream`)
+    bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
     // per-pass event timing (ss3d_context_profile)
@@ -200,8 +235,6 @@ struct ss3d_context {
     std::map<std::pair<int, int>, int> tune;
     std::vector<hipEvent_t> event_pool;
     ~ss3d_context() {
-        for (auto e : {ev_fill[0], ev_fill[1], ev_rest[0], ev_rest[1], ev_join})
-            if (e) (void)hipEventDestroy(e);
         for (auto& r : prof) {
             (void)hipEventDestroy(r.e0);
             (void)hipEventDestroy(r.e1);
@@ -218,7 +251,7 @@ struct RngTables {
 struct ss3d_spectrum {
     ss3d_context* ctx = nullptr;
     int npx = 0, npy = 0, npz = 0, hx = 0, hxa = 0;
-    DevBuf amp64, amp32h;
+    DevBuf amp64, amp32h, diagd;
     RngTables rng[2];
     ss3d_spec_diag diag{};
 };
@@ -473,13 +506,8 @@ int ss3d_context_create(int device, ss3d_context** out) {
     HIP_TRY(hipSetDevice(device));
     std::unique_ptr<ss3d_context> c(new ss3d_context);
     c->device = device;
-    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-    for (int i = 0; i < 2; ++i) {
-        HIP_TRY(hipEventCreateWithFlags(&c->ev_fill[i], hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&c->ev_rest[i], hipEventDisableTiming));
-    }
-    HIP_TRY(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    SS3D_TRY(c->main.init());
+    c->stream = c->main.stream;
     SS3D_TRY(build_twiddles(c.get()));
     *out = c.release();
     return SS3D_OK;
@@ -488,19 +516,18 @@ int ss3d_context_create(int device, ss3d_context** out) {
 int ss3d_context_destroy(ss3d_context* ctx) {
     if (!ctx) return SS3D_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamSynchronize(ctx->stream2);
-    hipStream_t s = ctx->stream, s2 = ctx->stream2;
+    ctx->main.sync();
+    hipStream_t s = ctx->main.stream, s2 = ctx->main.stream2;
     delete ctx;
-    (void)hipStreamDestroy(s);
+    if (s) (void)hipStreamDestroy(s);
     if (s2) (void)hipStreamDestroy(s2);
     return SS3D_OK;
 }
 
 int ss3d_context_synchronize(ss3d_context* ctx) {
     if (!ctx) return fail(SS3D_EINVAL, "null context");
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream2));
+    HIP_TRY(hipStreamSynchronize(ctx->main.stream));
+    HIP_TRY(hipStreamSynchronize(ctx->main.stream2));
     return SS3D_OK;
 }
 
@@ -545,17 +572,17 @@ int64_t ss3d_realization_state(int iseed, int npx, int npy, int npz, int igauss,
     return pm_mul(seed_state(iseed), pm_pow(kPmA, e));
 }
 
-int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, double nugget, double dx, double dy,
-                         double dz, int npx, int npy, int npz, ss3d_spectrum** out, ss3d_spec_diag* diag) {
-    if (!c || !out) return fail(SS3D_EINVAL, "null argument");
-    *out = nullptr;
+// Enqueues the whole spectrum build on lane L (no host synchronisation); the diagnostics land in
+// sp->diagd on the device.
+static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, int nst, double nugget, double dx,
+                          double dy, double dz, int npx, int npy, int npz, std::unique_ptr<ss3d_spectrum>& sp) {
     SS3D_TRY(check_structures(st, nst));
     SS3D_TRY(check_grid(npx, npy, npz));
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
     if (std::max({lx, ly, lz}) > 13)
         return fail(SS3D_ETOOBIG, "periodic axis longer than 8192 is not supported by the FP64 spectrum passes");
     HIP_TRY(hipSetDevice(c->device));
-    std::unique_ptr<ss3d_spectrum> sp(new ss3d_spectrum);
+    sp.reset(new ss3d_spectrum);
     sp->ctx = c;
     sp->npx = npx;
     sp->npy = npy;
@@ -564,11 +591,11 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     sp->hxa = (sp->hx + 1) & ~1;
     const int64_t ntot = (int64_t)npx * npy * npz;
     const int64_t nh = (int64_t)sp->hxa * npy * npz;
-    SS3D_TRY(sp->amp64.ensure(ntot * sizeof(double), c->stream));
-    SS3D_TRY(sp->amp32h.ensure(nh * sizeof(float), c->stream));
-    HIP_TRY(hipMemsetAsync(sp->amp32h.p, 0, nh * sizeof(float), c->stream));  // padding column stays 0
-    SS3D_TRY(c->wa.ensure(ntot * 2 * sizeof(double), c->stream));
-    SS3D_TRY(c->diag.ensure(8 * sizeof(double), c->stream));
+    SS3D_TRY(sp->amp64.ensure(ntot * sizeof(double), L.stream));
+    SS3D_TRY(sp->amp32h.ensure(nh * sizeof(float), L.stream));
+    HIP_TRY(hipMemsetAsync(sp->amp32h.p, 0, nh * sizeof(float), L.stream));  // padding column stays 0
+    SS3D_TRY(L.wa.ensure(ntot * 2 * sizeof(double), L.stream));
+    SS3D_TRY(sp->diagd.ensure(4 * sizeof(double), L.stream));
 
     PassArgs a = base_args(sp.get());
     a.nst = nst;
@@ -596,10 +623,10 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     px.a.logn = lx;
     px.a.nrows = npy * npz;
     px.a.n_out = npx;
-    px.a.out = c->wa.p;
+    px.a.out = L.wa.p;
     px.a.out_row = npx;
     px.a.tw = tw_ptr(c->tw64f, lx, 16);
-    SS3D_TRY(run_pass(c, px, 1, c->stream, c->partial));
+    SS3D_TRY(run_pass(c, px, 1, L.stream, L.partial));
     Pass* last = &px;
     Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a, 1};
     if (has_y) {
@@ -608,12 +635,12 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
         py.a.n0 = npx;
         py.a.nthird = npz;
         py.a.n_out = npy;
-        py.a.in = c->wa.p;
-        py.a.out = c->wa.p;
+        py.a.in = L.wa.p;
+        py.a.out = L.wa.p;
         py.a.in_sA = py.a.out_sA = npx;
         py.a.in_sT = py.a.out_sT = (int64_t)npx * npy;
         py.a.tw = tw_ptr(c->tw64f, ly, 16);
-        SS3D_TRY(run_pass(c, py, 1, c->stream, c->partial));
+        SS3D_TRY(run_pass(c, py, 1, L.stream, L.partial));
         last = &py;
     }
     Pass pz{64, IN_LOAD, OUT_SPEC, strided_shape(lz, npx, 128, 16, c->target_e64, emax), 0, a, 2};
@@ -623,24 +650,39 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
         pz.a.n0 = npx;
         pz.a.nthird = npy;
         pz.a.n_out = npz;
-        pz.a.in = c->wa.p;
-        pz.a.out = c->wa.p;
+        pz.a.in = L.wa.p;
+        pz.a.out = L.wa.p;
         pz.a.in_sA = pz.a.out_sA = (int64_t)npx * npy;
         pz.a.in_sT = pz.a.out_sT = npx;
         pz.a.tw = tw_ptr(c->tw64f, lz, 16);
-        SS3D_TRY(run_pass(c, pz, 1, c->stream, c->partial));
+        SS3D_TRY(run_pass(c, pz, 1, L.stream, L.partial));
         last = &pz;
     }
-    hipError_t e = launch_diag_finalize(c->partial.as<double>(), last->nblocks, c->diag.as<double>(), c->stream);
+    hipError_t e = launch_diag_finalize(L.partial.as<double>(), last->nblocks, sp->diagd.as<double>(), L.stream);
     if (e != hipSuccess) return fail(SS3D_EHIP, "diag finalize: %s", hipGetErrorString(e));
+    sp->diag.c0 = c0;
+    return SS3D_OK;
+}
+
+// Copies the device diagnostics of a finished build into sp->diag (synchronous).
+static int spectrum_read_diag(ss3d_spectrum* sp, hipStream_t s) {
     double d4[4];
-    HIP_TRY(hipMemcpyAsync(d4, c->diag.p, sizeof d4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpyAsync(d4, sp->diagd.p, sizeof d4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     sp->diag.max_imag = d4[0];
     sp->diag.max_real = d4[1];
     sp->diag.sneg = d4[2];
     sp->diag.spos = d4[3];
-    sp->diag.c0 = c0;
+    return SS3D_OK;
+}
+
+int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, double nugget, double dx, double dy,
+                         double dz, int npx, int npy, int npz, ss3d_spectrum** out, ss3d_spec_diag* diag) {
+    if (!c || !out) return fail(SS3D_EINVAL, "null argument");
+    *out = nullptr;
+    std::unique_ptr<ss3d_spectrum> sp;
+    SS3D_TRY(spectrum_build(c, c->main, st, nst, nugget, dx, dy, dz, npx, npy, npz, sp));
+    SS3D_TRY(spectrum_read_diag(sp.get(), c->main.stream));
     if (diag) *diag = sp->diag;
     *out = sp.release();
     return SS3D_OK;
@@ -649,8 +691,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
 int ss3d_context_profile(ss3d_context* c, int enable) {
     if (!c) return fail(SS3D_EINVAL, "null context");
     if (enable) {
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream2));
+        c->main.sync();
         for (auto& r : c->prof) {
             c->event_pool.push_back(r.e0);
             c->event_pool.push_back(r.e1);
@@ -663,8 +704,7 @@ int ss3d_context_profile(ss3d_context* c, int enable) {
 
 int ss3d_context_profile_read(ss3d_context* c, double* ms, double* bytes, int64_t* launches, int nkinds) {
     if (!c || !ms || !bytes || !launches || nkinds < 1) return fail(SS3D_EINVAL, "null argument");
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream2));
+    c->main.sync();
     for (int k = 0; k < nkinds; ++k) {
         ms[k] = 0;
         bytes[k] = 0;
@@ -717,16 +757,16 @@ int tuned_rm(const ss3d_context* c, int kind, int L) {
 }
 
 // Enqueue realizations r0..r0+nb-1 into dout (device) and their statistics into dstats.
-int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nb, int nx, int ny,
+int enqueue_batch(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nb, int nx, int ny,
                   int nz, int prec, void* dout, double* dstats) {
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz, hx = sp->hx;
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
     SS3D_TRY(build_rng(sp, igauss));
-    SS3D_TRY(c->rbase.ensure((size_t)nb * 4, c->stream));
+    SS3D_TRY(L.rbase.ensure((size_t)nb * 4, L.stream));
     {
         const uint64_t dmod = (uint64_t)draws_per_realization(npx, npy, npz, igauss) % (kPmM - 1);
-        hipLaunchKernelGGL(rbase_kernel, dim3((nb + 63) / 64), dim3(64), 0, c->stream, seed_state(iseed), dmod,
-                           (uint64_t)r0, nb, c->rbase.as<uint32_t>());
+        hipLaunchKernelGGL(rbase_kernel, dim3((nb + 63) / 64), dim3(64), 0, L.stream, seed_state(iseed), dmod,
+                           (uint64_t)r0, nb, L.rbase.as<uint32_t>());
         HIP_TRY(hipGetLastError());
     }
     PassArgs a = base_args(sp);
@@ -738,7 +778,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
     a.amp32h = sp->amp32h.as<float>();
     a.rowf = sp->rng[igauss].rowf.as<uint32_t>();
     a.txp = sp->rng[igauss].txp.as<uint32_t>();
-    a.rbase = c->rbase.as<uint32_t>();
+    a.rbase = L.rbase.as<uint32_t>();
     const bool has_y = npy > 1, has_z = npz > 1;
     const int64_t ncrop = (int64_t)nx * ny * nz;
     Pass* last = nullptr;
@@ -747,8 +787,8 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         // x -> y -> z, C2C, pruned to the crop after each pass (f90:1335-1342)
         const int emax = 8192;
         const int64_t b1 = (int64_t)nx * npy * npz, b2 = (int64_t)nx * ny * npz;
-        SS3D_TRY(c->wa.ensure((size_t)b1 * nb * 16, c->stream));
-        SS3D_TRY(c->wb.ensure((size_t)std::max<int64_t>(b2, 1) * nb * 16, c->stream));
+        SS3D_TRY(L.wa.ensure((size_t)b1 * nb * 16, L.stream));
+        SS3D_TRY(L.wb.ensure((size_t)std::max<int64_t>(b2, 1) * nb * 16, L.stream));
         p1 = Pass{64, IN_FILL64, (has_y || has_z) ? OUT_STORE : OUT_REAL,
                   contig_shape(lx, (int64_t)npy * npz, c->target_e64, emax), 0, a, 3};
         p1.a.axis = 0;
@@ -757,7 +797,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
         p1.a.n_out = nx;
         p1.a.tw = tw_ptr(c->tw64i, lx, 16);
         if (has_y || has_z) {
-            p1.a.out = c->wa.p;
+            p1.a.out = L.wa.p;
             p1.a.out_row = nx;
             p1.a.out_bstride = b1;
         } else {
@@ -765,7 +805,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p1.a.out_row = nx;
             p1.a.out_bstride = ncrop;
         }
-        SS3D_TRY(run_pass(c, p1, nb, c->stream, c->partial));
+        SS3D_TRY(run_pass(c, p1, nb, L.stream, L.partial));
         last = &p1;
         if (has_y) {
             p2 = Pass{64, IN_LOAD, has_z ? OUT_STORE : OUT_REAL, strided_shape(ly, nx, 128, 16, c->target_e64, emax), 0, a, 4};
@@ -774,16 +814,16 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p2.a.n0 = nx;
             p2.a.nthird = npz;
             p2.a.n_out = ny;
-            p2.a.in = c->wa.p;
+            p2.a.in = L.wa.p;
             p2.a.in_sA = nx;
             p2.a.in_sT = (int64_t)nx * npy;
             p2.a.in_bstride = b1;
-            p2.a.out = has_z ? c->wb.p : dout;
+            p2.a.out = has_z ? L.wb.p : dout;
             p2.a.out_sA = nx;
             p2.a.out_sT = (int64_t)nx * ny;
             p2.a.out_bstride = has_z ? b2 : ncrop;
             p2.a.tw = tw_ptr(c->tw64i, ly, 16);
-            SS3D_TRY(run_pass(c, p2, nb, c->stream, c->partial));
+            SS3D_TRY(run_pass(c, p2, nb, L.stream, L.partial));
             last = &p2;
         }
         if (has_z) {
@@ -793,7 +833,7 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p3.a.n0 = nx;
             p3.a.nthird = ny;
             p3.a.n_out = nz;
-            p3.a.in = has_y ? c->wb.p : c->wa.p;
+            p3.a.in = has_y ? L.wb.p : L.wa.p;
             p3.a.in_sA = (int64_t)nx * ny;
             p3.a.in_sT = nx;
             p3.a.in_bstride = has_y ? b2 : b1;
@@ -802,41 +842,41 @@ int enqueue_batch(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int
             p3.a.out_sT = nx;
             p3.a.out_bstride = ncrop;
             p3.a.tw = tw_ptr(c->tw64i, lz, 16);
-            SS3D_TRY(run_pass(c, p3, nb, c->stream, c->partial));
+            SS3D_TRY(run_pass(c, p3, nb, L.stream, L.partial));
             last = &p3;
         }
     } else {
         return fail(SS3D_EINVAL, "enqueue_batch handles the FP64 path only");
     }
-    hipError_t e = launch_stats_finalize(c->partial.as<double>(), last->nblocks, nb, (double)ncrop, dstats, c->stream);
+    hipError_t e = launch_stats_finalize(L.partial.as<double>(), last->nblocks, nb, (double)ncrop, dstats, L.stream);
     if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
     return SS3D_OK;
 }
 
 // Grows a buffer shared by both streams (waits for both before freeing the old one).
-int ensure2(ss3d_context* c, DevBuf& b, size_t bytes) {
+int ensure2(Lane& L, DevBuf& b, size_t bytes) {
     if (bytes <= b.cap) return SS3D_OK;
-    HIP_TRY(hipStreamSynchronize(c->stream2));
-    return b.ensure(bytes, c->stream);
+    HIP_TRY(hipStreamSynchronize(L.stream2));
+    return b.ensure(bytes, L.stream);
 }
 
 // FP32 realizations r0..r0+nreal-1: z -> y passes over the half spectrum (column 0 also carries
 // the Nyquist column), then the C2R x pass, in batches of nb realizations.  The fill pass (the
 // first one) runs on c->stream and writes one of two pipe buffers; the other passes and the
-// statistics run on c->stream2, so the fill of batch k+1 (arithmetic-bound) overlaps the
+// statistics run on L.stream2, so the fill of batch k+1 (arithmetic-bound) overlaps the
 // memory-bound passes of batch k.
-int enqueue_fp32(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nreal, int nx, int ny,
+int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nreal, int nx, int ny,
                  int nz, void* dout, double* dstats) {
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
     if (lx > kMaxFastLog || ly > kMaxFastLog || lz > kMaxFastLog)
         return fail(SS3D_ETOOBIG, "periodic axis longer than %d", 1 << kMaxFastLog);
     SS3D_TRY(build_rng(sp, igauss));
-    SS3D_TRY(ensure2(c, c->rbase, (size_t)nreal * 4));
+    SS3D_TRY(ensure2(L, L.rbase, (size_t)nreal * 4));
     {
         const uint64_t dmod = (uint64_t)draws_per_realization(npx, npy, npz, igauss) % (kPmM - 1);
-        hipLaunchKernelGGL(rbase_kernel, dim3((nreal + 63) / 64), dim3(64), 0, c->stream, seed_state(iseed), dmod,
-                           (uint64_t)r0, nreal, c->rbase.as<uint32_t>());
+        hipLaunchKernelGGL(rbase_kernel, dim3((nreal + 63) / 64), dim3(64), 0, L.stream, seed_state(iseed), dmod,
+                           (uint64_t)r0, nreal, L.rbase.as<uint32_t>());
         HIP_TRY(hipGetLastError());
     }
     PassArgs a = base_args(sp);
@@ -896,15 +936,15 @@ int enqueue_fp32(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int6
 
     const bool staged = has_y || has_z;  // a fill pass followed by further passes
     const int64_t pipe_elems = has_z ? bz : by;  // output of the fill pass
-    hipStream_t s2 = c->pipeline ? c->stream2 : c->stream;  // stream of the passes after the fill
+    hipStream_t s2 = c->pipeline ? L.stream2 : L.stream;  // stream of the passes after the fill
     const bool pipelined = staged && c->pipeline;
     if (staged) {
-        for (int i = 0; i < 2; ++i) SS3D_TRY(ensure2(c, c->pipe[i], (size_t)std::max<int64_t>(pipe_elems, 1) * nb * 8));
-        if (has_z && has_y) SS3D_TRY(ensure2(c, c->wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
+        for (int i = 0; i < 2; ++i) SS3D_TRY(ensure2(L, L.pipe[i], (size_t)std::max<int64_t>(pipe_elems, 1) * nb * 8));
+        if (has_z && has_y) SS3D_TRY(ensure2(L, L.wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
     }
     if (pipelined) {
-        HIP_TRY(hipEventRecord(c->ev_join, c->stream));
-        HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_join, 0));
+        HIP_TRY(hipEventRecord(L.ev_join, L.stream));
+        HIP_TRY(hipStreamWaitEvent(L.stream2, L.ev_join, 0));
     }
     const int nbat = (nreal + nb - 1) / nb;
     for (int k = 0; k < nbat; ++k) {
@@ -912,42 +952,42 @@ int enqueue_fp32(ss3d_context* c, ss3d_spectrum* sp, int iseed, int igauss, int6
         const int buf = k & 1;
         void* out = static_cast<char*>(dout) + (size_t)done * ncrop * 4;
         double* st = dstats + 3 * (size_t)done;
-        const uint32_t* rb = c->rbase.as<uint32_t>() + done;
+        const uint32_t* rb = L.rbase.as<uint32_t>() + done;
         if (!staged) {
             px.a.rbase = rb;
             px.a.out = out;
-            SS3D_TRY(run_pass(c, px, cnt, c->stream, c->partial));
-            hipError_t e = launch_stats_finalize(c->partial.as<double>(), px.nblocks, cnt, (double)ncrop, st, c->stream);
+            SS3D_TRY(run_pass(c, px, cnt, L.stream, L.partial));
+            hipError_t e = launch_stats_finalize(L.partial.as<double>(), px.nblocks, cnt, (double)ncrop, st, L.stream);
             if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
             continue;
         }
-        if (pipelined && k >= 2) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_rest[buf], 0));
+        if (pipelined && k >= 2) HIP_TRY(hipStreamWaitEvent(L.stream, L.ev_rest[buf], 0));
         Pass& fill = has_z ? pz : py;
         fill.a.rbase = rb;
-        fill.a.out = c->pipe[buf].p;
-        SS3D_TRY(run_pass(c, fill, cnt, c->stream, c->partial));
+        fill.a.out = L.pipe[buf].p;
+        SS3D_TRY(run_pass(c, fill, cnt, L.stream, L.partial));
         if (pipelined) {
-            HIP_TRY(hipEventRecord(c->ev_fill[buf], c->stream));
-            HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fill[buf], 0));
+            HIP_TRY(hipEventRecord(L.ev_fill[buf], L.stream));
+            HIP_TRY(hipStreamWaitEvent(L.stream2, L.ev_fill[buf], 0));
         }
-        DevBuf& part = pipelined ? c->partial2 : c->partial;
+        DevBuf& part = pipelined ? L.partial2 : L.partial;
         if (has_z && has_y) {
-            py.a.in = c->pipe[buf].p;
-            py.a.out = c->wb.p;
+            py.a.in = L.pipe[buf].p;
+            py.a.out = L.wb.p;
             SS3D_TRY(run_pass(c, py, cnt, s2, part));
-            px.a.in = c->wb.p;
+            px.a.in = L.wb.p;
         } else {
-            px.a.in = c->pipe[buf].p;
+            px.a.in = L.pipe[buf].p;
         }
         px.a.out = out;
         SS3D_TRY(run_pass(c, px, cnt, s2, part));
         hipError_t e = launch_stats_finalize(part.as<double>(), px.nblocks, cnt, (double)ncrop, st, s2);
         if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
-        if (pipelined) HIP_TRY(hipEventRecord(c->ev_rest[buf], c->stream2));
+        if (pipelined) HIP_TRY(hipEventRecord(L.ev_rest[buf], L.stream2));
     }
     if (pipelined) {
-        HIP_TRY(hipEventRecord(c->ev_join, c->stream2));
-        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        HIP_TRY(hipEventRecord(L.ev_join, L.stream2));
+        HIP_TRY(hipStreamWaitEvent(L.stream, L.ev_join, 0));
     }
     return SS3D_OK;
 }
@@ -992,17 +1032,17 @@ int ss3d_realize_device(ss3d_context* c, const ss3d_spectrum* spc, int iseed, in
     ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
     const int64_t ncrop = (int64_t)nx * ny * nz;
     if (prec == SS3D_FP32) {
-        if (!dev_stats) SS3D_TRY(ensure2(c, c->stats, (size_t)nreal * 3 * sizeof(double)));
-        return enqueue_fp32(c, sp, iseed, igauss, r0, nreal, nx, ny, nz, dev_out,
-                            dev_stats ? dev_stats : c->stats.as<double>());
+        if (!dev_stats) SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)nreal * 3 * sizeof(double)));
+        return enqueue_fp32(c, c->main, sp, iseed, igauss, r0, nreal, nx, ny, nz, dev_out,
+                            dev_stats ? dev_stats : c->main.stats.as<double>());
     }
     const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
-    if (!dev_stats) SS3D_TRY(ensure2(c, c->stats, (size_t)nreal * 3 * sizeof(double)));
+    if (!dev_stats) SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)nreal * 3 * sizeof(double)));
     for (int done = 0; done < nreal; done += nb) {
         const int cnt = std::min(nb, nreal - done);
         void* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * 8;
-        double* st = (dev_stats ? dev_stats : c->stats.as<double>()) + 3 * (size_t)done;
-        SS3D_TRY(enqueue_batch(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, out, st));
+        double* st = (dev_stats ? dev_stats : c->main.stats.as<double>()) + 3 * (size_t)done;
+        SS3D_TRY(enqueue_batch(c, c->main, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, out, st));
     }
     return SS3D_OK;
 }
@@ -1020,16 +1060,16 @@ int ss3d_realize(ss3d_context* c, const ss3d_spectrum* spc, int iseed, int igaus
     const int nb = batch_size(sp, nx, ny, nz, prec, nreal);
     const int64_t by_mem = std::max<int64_t>(1, (int64_t(1) << 30) / std::max<int64_t>(ncrop * (int64_t)esz, 1));
     const int ch = (int)std::min<int64_t>(nreal, std::max<int64_t>(nb, by_mem));
-    SS3D_TRY(ensure2(c, c->dout, (size_t)ch * ncrop * esz));
-    SS3D_TRY(ensure2(c, c->stats, (size_t)ch * 3 * sizeof(double)));
+    SS3D_TRY(ensure2(c->main, c->main.dout, (size_t)ch * ncrop * esz));
+    SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)ch * 3 * sizeof(double)));
     std::vector<double> hs((size_t)ch * 3);
     for (int done = 0; done < nreal; done += ch) {
         const int cnt = std::min(ch, nreal - done);
-        SS3D_TRY(ss3d_realize_device(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, c->dout.p,
-                                     c->stats.as<double>()));
-        HIP_TRY(hipMemcpyAsync(static_cast<char*>(host_out) + (size_t)done * ncrop * esz, c->dout.p,
+        SS3D_TRY(ss3d_realize_device(c, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, c->main.dout.p,
+                                     c->main.stats.as<double>()));
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(host_out) + (size_t)done * ncrop * esz, c->main.dout.p,
                                (size_t)cnt * ncrop * esz, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(hs.data(), c->stats.p, (size_t)cnt * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(hs.data(), c->main.stats.p, (size_t)cnt * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         for (int i = 0; i < cnt; ++i) {
             if (mean) mean[done + i] = hs[3 * i];
@@ -1052,17 +1092,17 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
     const int Lk[3] = {lz, ly, sp->npx >= 2 ? lx - 1 : 0};
     const int nb = batch_size(sp, nx, ny, nz, SS3D_FP32, 64);
     const int64_t ncrop = (int64_t)nx * ny * nz;
-    SS3D_TRY(ensure2(c, c->dout, (size_t)nb * ncrop * 4));
-    SS3D_TRY(ensure2(c, c->stats, (size_t)nb * 3 * sizeof(double)));
+    SS3D_TRY(ensure2(c->main, c->main.dout, (size_t)nb * ncrop * 4));
+    SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)nb * 3 * sizeof(double)));
     const bool was = c->profiling;
     double best[3] = {1e30, 1e30, 1e30};
     int pick[3] = {5, 5, 5};
     for (int rm = 3; rm <= 5; ++rm) {
         for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = rm;
-        SS3D_TRY(enqueue_fp32(c, sp, 1, igauss, 0, nb, nx, ny, nz, c->dout.p, c->stats.as<double>()));
+        SS3D_TRY(enqueue_fp32(c, c->main, sp, 1, igauss, 0, nb, nx, ny, nz, c->main.dout.p, c->main.stats.as<double>()));
         SS3D_TRY(ss3d_context_profile(c, 1));
         for (int r = 0; r < std::max(1, reps); ++r)
-            SS3D_TRY(enqueue_fp32(c, sp, 1, igauss, 0, nb, nx, ny, nz, c->dout.p, c->stats.as<double>()));
+            SS3D_TRY(enqueue_fp32(c, c->main, sp, 1, igauss, 0, nb, nx, ny, nz, c->main.dout.p, c->main.stats.as<double>()));
         double ms[SS3D_NPASS_KINDS], by[SS3D_NPASS_KINDS];
         int64_t nl[SS3D_NPASS_KINDS];
         SS3D_TRY(ss3d_context_profile_read(c, ms, by, nl, SS3D_NPASS_KINDS));
@@ -1081,7 +1121,7 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
     if (!c) return fail(SS3D_EINVAL, "null context");
     if (option == SS3D_OPT_PIPELINE) {
         HIP_TRY(hipStreamSynchronize(c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream2));
+        HIP_TRY(hipStreamSynchronize(c->main.stream2));
         c->pipeline = value != 0;
         return SS3D_OK;
     }
@@ -1102,3 +1142,5 @@ int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
 }
 
 }  // extern "C"
+
+#include "ss3d_server.inc"

@@ -14,7 +14,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 
 def declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|int64_t)\s+(ss3d_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|double)\s+(ss3d_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
