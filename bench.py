@@ -141,6 +141,50 @@ def serve(S, eng, rate, duration, seed, nreal=64, nslots=4, spot_frac=0.05, repe
             "spot_checks": len(spots), "spot_max_rel_rms": max(spots) if spots else None}
 
 
+def uneven_batch(S, eng, seed, njobs=32, nlanes=4, reps=5):
+    """32 FP32 jobs of uneven grids (nx, ny in {32,64,128,256}, nz in {1,16,64,128}; spherical,
+    exponential or gaussian model, ranges U(5,100) per axis, azimuth U(0,180), dip U(-30,30),
+    nugget U(0,0.3), sill 1 - nugget, nreal U{1..16}, ipad=1) captured into one HIP graph on
+    `nlanes` concurrent branches.  Creation (spectral densities, per-grid autotuning, one eager run,
+    capture) is untimed; timed: `reps` graph replays, then `reps` eager enqueues of the same passes."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(njobs):
+        nx, ny = (int(v) for v in rng.choice([32, 64, 128, 256], 2))
+        nz = int(rng.choice([1, 16, 64, 128]))
+        nug = float(rng.uniform(0.0, 0.3))
+        st = [(int(rng.integers(1, 4)), 1.0 - nug, *(float(v) for v in rng.uniform(5, 100, 3)),
+               float(rng.uniform(0, 180)), float(rng.uniform(-30, 30)), 0.0)]
+        reqs.append(S.Server.request(i, st, nug, 1, 1, 1, nx, ny, nz, 1, int(rng.integers(1, 2**31 - 1)),
+                                     int(rng.integers(1, 17)), 1, S.FP32))
+    t0 = time.perf_counter()
+    b = S.Batch(eng, reqs, nlanes=nlanes)
+    eng.synchronize()
+    t_create = time.perf_counter() - t0
+    try:
+        out = {}
+        for mode, graph in (("graph", True), ("eager", False)):
+            b.run(graph=graph)
+            eng.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                b.run(graph=graph)
+            eng.synchronize()
+            out[mode] = (time.perf_counter() - t0) / reps
+    finally:
+        b.close()
+    return {"jobs": njobs, "lanes": nlanes, "cells_per_batch": b.cells, "reps": reps,
+            "graph_gcells_s": round(b.cells / out["graph"] / 1e9, 3), "graph_ms": round(out["graph"] * 1e3, 3),
+            "eager_gcells_s": round(b.cells / out["eager"] / 1e9, 3), "eager_ms": round(out["eager"] * 1e3, 3),
+            "create_s": round(t_create, 2),
+            "distinct_periodic_grids": len({S.periodic_grid([(r.st[0].itype, r.st[0].sill, r.st[0].ahmax,
+                                                              r.st[0].ahmin, r.st[0].avert, r.st[0].az, r.st[0].dip,
+                                                              r.st[0].plg)], 1, 1, 1, r.nx, r.ny, r.nz, 1)
+                                             for r in reqs})}
+
+
 def load_traffic(kind):
     path = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if not os.path.exists(path):
@@ -161,6 +205,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -218,6 +263,7 @@ def main():
     hip.free(stats)
     spec.close()
 
+    uneven = None if args.no_batch else uneven_batch(S, eng, 3)
     serving = None
     if not args.no_serve:
         sustained = serve(S, eng, args.serve_rate, 6.0, 1)
@@ -257,6 +303,7 @@ def main():
         "fp32_over_fp64": round(value / fp64_value, 2),
         "spectrum_ms": round(spec_ms, 2),
         "autotune": {"largest_radix_log2": plan, "seconds": round(tune_s, 2)},
+        "uneven_batch": uneven,
         "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
                         "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

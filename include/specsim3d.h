@@ -172,6 +172,23 @@ double ss3d_server_clock(const ss3d_server* s);
 /* Waits for every submitted request, then releases the lanes and the cache. */
 int ss3d_server_destroy(ss3d_server* s);
 
+/* ---------------------------------------------------------------- batch graph
+ * No counterpart in the reference.  Many FP32 jobs of uneven grids (ss3d_request, host_out
+ * ignored), simulated together: creation builds their spectral densities (shared between
+ * identical models and grids), autotunes every distinct grid, runs every job once eagerly to
+ * size device buffers, and captures all jobs' passes into one HIP graph, jobs spread over
+ * `nlanes` (1..16) concurrent branches.  ss3d_batch_run replays it on the context's stream
+ * (use_graph = 1) or enqueues every pass again (use_graph = 0), with the same results; it
+ * returns before the work ends (ss3d_context_synchronize waits).  The results of job i stay on
+ * the device until ss3d_batch_output copies them (nreal*nx*ny*nz floats, x fastest; per
+ * realization mean, variance and 0).  The context must not be used by other threads meanwhile. */
+typedef struct ss3d_batch ss3d_batch;
+int ss3d_batch_create(ss3d_context* ctx, const ss3d_request* jobs, int njobs, int nlanes, ss3d_batch** out);
+int ss3d_batch_run(ss3d_batch* b, int use_graph);
+int ss3d_batch_output(ss3d_batch* b, int job, void* host_out, double* host_stats);
+int64_t ss3d_batch_cells(const ss3d_batch* b);
+int ss3d_batch_destroy(ss3d_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

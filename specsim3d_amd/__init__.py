@@ -99,6 +99,11 @@ SIGNATURES = {
     "ss3d_server_info": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_int64)] * 4),
     "ss3d_server_clock": (ctypes.c_double, [_vp]),
     "ss3d_server_destroy": (ctypes.c_int, [_vp]),
+    "ss3d_batch_create": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ss3d_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ss3d_batch_output": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ss3d_batch_cells": (ctypes.c_int64, [_vp]),
+    "ss3d_batch_destroy": (ctypes.c_int, [_vp]),
     "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
 }
 
@@ -328,6 +333,42 @@ class Server:
     def close(self):
         if self.handle:
             lib().ss3d_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many FP32 jobs of uneven grids captured into one HIP graph (ss3d_batch_* in
+    include/specsim3d.h).  `requests` are Server.request records; host_out is ignored."""
+
+    def __init__(self, engine, requests, nlanes=4):
+        self.engine = engine
+        self.requests = list(requests)
+        arr = (Request * len(self.requests))(*self.requests)
+        self.handle = _vp()
+        check(lib().ss3d_batch_create(engine.handle, arr, len(self.requests), nlanes, ctypes.byref(self.handle)))
+        self.cells = lib().ss3d_batch_cells(self.handle)
+
+    def run(self, graph=True):
+        """Enqueue the whole batch (one graph launch, or every pass again); returns at once."""
+        check(lib().ss3d_batch_run(self.handle, 1 if graph else 0))
+
+    def output(self, i):
+        """(field[r, z, y, x] float32, stats[r, 3]) of job i, after the work has ended."""
+        r = self.requests[i]
+        f = np.empty((r.nreal, r.nz, r.ny, r.nx), dtype=np.float32)
+        st = np.empty((r.nreal, 3), dtype=np.float64)
+        check(lib().ss3d_batch_output(self.handle, i, f.ctypes.data, st.ctypes.data))
+        return f, st
+
+    def close(self):
+        if self.handle:
+            lib().ss3d_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

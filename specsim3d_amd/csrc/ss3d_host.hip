@@ -1144,3 +1144,4 @@ int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
 }  // extern "C"
 
 #include "ss3d_server.inc"
+#include "ss3d_batch.inc"
