@@ -261,6 +261,11 @@ struct StridedCfg {
 
 __device__ __forceinline__ float4 pack2(f2 a, f2 b) { return make_float4(a.x, a.y, b.x, b.y); }
 
+// Fill mode (IN_FILL32H): a block serves realizations b0 .. b1-1 of its tile (a.bpb per block).
+// What depends on the site and not on the realization -- the amplitude, the product of the
+// site's row factor and column factor, the conjugation and self-conjugate flags -- is read and
+// computed once and kept in registers; per realization only its base factor is multiplied in
+// (the Park-Miller state is a product mod 2^31-1, so the order of the factors does not matter).
 template <int L, int P, int IN, int AXIS, int RM>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(PassArgs a) {
     using C = StridedCfg<L, P, RM>;
@@ -273,13 +278,63 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
     const int tile = blockIdx.x;
     const int i0base = (tile % a.ntiles0) * P;
     const int third = tile / a.ntiles0;
-    const int b = blockIdx.y;
     const int c0 = i0base + 2 * q;
     const bool active = c0 < a.n0;
     f2 v0[EPT], v1[EPT];
 
-    // pass-0 operands: positions g + r*G
-    if constexpr (IN == IN_LOAD) {
+    // radix passes (LDS exchange between them), then the store of batch member b:
+    // butterfly g + G*i, element r at out_pos; positions >= n_out pruned
+    auto transform_store = [&](int b, const int g, const int q) {
+        [&]<int... J>(std::integer_sequence<int, J...>) {
+            (
+                [&] {
+                    constexpr int Jc = J;
+                    if constexpr (Jc > 0) {
+                        constexpr int LRp = rlog(L, Jc - 1, RM), NBp = EPT >> LRp;
+                        constexpr int LR = rlog(L, Jc, RM), NB = EPT >> LR;
+                        if constexpr (Jc > 1) __syncthreads();
+#pragma unroll
+                        for (int i = 0; i < NBp; ++i)
+#pragma unroll
+                            for (int r = 0; r < (1 << LRp); ++r) {
+                                const int e = i * (1 << LRp) + r;
+                                lds4[C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r)) * PH + q] = pack2(v0[e], v1[e]);
+                            }
+                        __syncthreads();
+#pragma unroll
+                        for (int i = 0; i < NB; ++i)
+#pragma unroll
+                            for (int r = 0; r < (1 << LR); ++r) {
+                                const int e = i * (1 << LR) + r;
+                                const float4 w = lds4[C::ppos(g + G * i + r * (N >> LR)) * PH + q];
+                                v0[e] = mk(w.x, w.y);
+                                v1[e] = mk(w.z, w.w);
+                            }
+                    }
+                    radix_pass<L, Jc, G, RM>(v0, g, tw);
+                    radix_pass<L, Jc, G, RM>(v1, g, tw);
+                }(),
+                ...);
+        }(std::make_integer_sequence<int, M>{});
+        if (active) {
+            float4* __restrict__ dst = reinterpret_cast<float4*>(
+                reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base);
+            const uint32_t sA = (uint32_t)a.out_sA >> 1;
+            constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
+            const int nout = a.n_out;
+#pragma unroll
+            for (int i = 0; i < NBl; ++i)
+#pragma unroll
+                for (int r = 0; r < (1 << LRl); ++r) {
+                    const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
+                    const int e = i * (1 << LRl) + r;
+                    if (pos < nout) dst[q + (uint32_t)pos * sA] = pack2(v0[e], v1[e]);
+                }
+        }
+    };
+
+    if constexpr (IN == IN_LOAD) {  // pass-0 operands at positions g + r*G
+        const int b = blockIdx.y;
         const float4* __restrict__ src = reinterpret_cast<const float4*>(
             reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride + (int64_t)third * a.in_sT + i0base);
         const uint32_t sA = (uint32_t)a.in_sA >> 1;  // in float4 units
@@ -295,91 +350,79 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
 #pragma unroll
             for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
         }
+        transform_store(b, g, q);
     } else {  // IN_FILL32H: AXIS 2: (ky, kz) = (third, pos), AXIS 1: (pos, third)
+        const int b0 = blockIdx.y * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
+        // Per element and column: row factor x column factor of the site, bit 31 set when the
+        // site is the partner (conjugate).  A site is self-conjugate only in column 0 (kx = 0,
+        // and the Nyquist column it carries) on a self row (row == partner row): selfm bit r.
+        uint32_t w0[EPT], w1[EPT];
+        uint32_t selfm = 0;
+        const bool col0 = active && c0 == 0, pack = col0 && a.npx >= 2;
+        const int kxn = a.npx >> 1;
+        uint32_t txn = 0;
+        // amplitude of element r: amp32h[c0 + hxa * row(pos)], row affine in r
+        const int64_t arow0 = AXIS == 2 ? (int64_t)third + (int64_t)a.npy * g : (int64_t)g + (int64_t)a.npy * third;
+        const int64_t arstep = AXIS == 2 ? (int64_t)a.npy * G : G;
+        const float* __restrict__ abase = a.amp32h + c0 + a.hxa * arow0;
+        const int64_t astep = a.hxa * arstep;
         if (active) {
-            const uint32_t rb = __ldg(a.rbase + b);
-            const Column ca = column_of(a, rb, c0), cb = column_of(a, rb, c0 + 1);
-            const bool pack = c0 == 0 && a.npx >= 2;  // column 0 also carries the Nyquist column
-            uint32_t rowv[EPT];
-            float2 ampv[EPT];
-            Site sv[EPT];
+            const int kxa = c0, kxb = c0 + 1;
+            const int pa = (a.npx - kxa) & (a.npx - 1), pb = (a.npx - kxb) & (a.npx - 1);
+            const uint32_t txa = __ldg(a.txp + kxa), txpa = __ldg(a.txp + pa);
+            const uint32_t txb = __ldg(a.txp + kxb), txpb = __ldg(a.txp + pb);
+            if (pack) txn = __ldg(a.txp + kxn);
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
                 const int pos = g + r * G;
-                sv[r] = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
-                rowv[r] = __ldg(a.rowf + sv[r].site);
-                ampv[r] = __ldg(reinterpret_cast<const float2*>(a.amp32h + c0 + a.hxa * sv[r].row));
+                const Site sv = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
+                const uint32_t rf = __ldg(a.rowf + sv.site);
+                // the site row is shared by every column; x order decides only inside self rows
+                const bool ja = sv.gt || (sv.eq && kxa > pa);
+                const bool jb = sv.gt || (sv.eq && kxb > pb);
+                w0[r] = pm_mul(rf, ja ? txpa : txa) | uint32_t(ja) << 31;
+                w1[r] = pm_mul(rf, jb ? txpb : txb) | uint32_t(jb) << 31;
+                if (sv.eq) selfm |= 1u << r;
             }
-#pragma unroll
-            for (int r = 0; r < EPT; ++r) {
-                // the site row is shared by the pair; x order decides only inside self rows
-                const bool j0 = site_conj(sv[r], ca), j1 = site_conj(sv[r], cb);
-                v0[r] = coef(ampv[r].x, pm_mul(rowv[r], j0 ? ca.tpar : ca.town), a.igauss, j0, site_self(sv[r], ca));
-                v1[r] = coef(ampv[r].y, pm_mul(rowv[r], j1 ? cb.tpar : cb.town), a.igauss, j1, site_self(sv[r], cb));
-            }
-            if (pack) {
-                const Column cn = column_of(a, rb, a.npx >> 1);
+        }
+        for (int b = b0; b < b1; ++b) {
+            if (b > b0) __syncthreads();  // LDS reads of the previous member are done
+            if (active) {
+                const uint32_t rb = __ldg(a.rbase + b);
+                // keep the site values and the amplitude base opaque so that nothing derived from
+                // them is hoisted out of the member loop (registers); the amplitudes are re-read (L2)
+                const float* ab = abase;
+                asm volatile("" : "+v"(ab));
 #pragma unroll
                 for (int r = 0; r < EPT; ++r) {
-                    const int pos = g + r * G;
-                    const f2 zn = AXIS == 2 ? fill_at(a, cn, third, pos) : fill_at(a, cn, pos, third);
-                    v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
+                    asm volatile("" : "+v"(w0[r]), "+v"(w1[r]));
+                    const float2 am = __ldg(reinterpret_cast<const float2*>(ab + r * astep));
+                    v0[r] = coef(am.x, pm_mul(w0[r] & 0x7fffffffu, rb), a.igauss, w0[r] >> 31, false);
+                    v1[r] = coef(am.y, pm_mul(w1[r] & 0x7fffffffu, rb), a.igauss, w1[r] >> 31, false);
                 }
-            }
-        } else {
+                if (col0) {  // kx = 0: self-conjugate sites; the Nyquist column (kx = npx/2, its own partner)
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
+                    for (int r = 0; r < EPT; ++r) {
+                        const uint32_t st = pm_mul(w0[r] & 0x7fffffffu, rb);  // column factor of kx = 0 is 1
+                        const bool cj = w0[r] >> 31, sf = (selfm >> r) & 1u;
+                        const float* ap = ab + r * astep;
+                        if (sf) v0[r] = coef(__ldg(ap), st, a.igauss, false, true);
+                        if (pack) {
+                            const f2 zn = coef(__ldg(ap + kxn), pm_mul(st, txn), a.igauss, cj, sf);
+                            v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
+            }
+            // per-member recomputation of the thread's LDS, twiddle and store addresses (opaque
+            // thread indices) instead of keeping them in registers across the member loop
+            int gg = g, qq = q;
+            asm volatile("" : "+v"(gg), "+v"(qq));
+            transform_store(b, gg, qq);
         }
-    }
-
-    [&]<int... J>(std::integer_sequence<int, J...>) {
-        (
-            [&] {
-                constexpr int Jc = J;
-                if constexpr (Jc > 0) {
-                    // exchange: previous pass's outputs -> this pass's operands
-                    constexpr int LRp = rlog(L, Jc - 1, RM), NBp = EPT >> LRp;
-                    constexpr int LR = rlog(L, Jc, RM), NB = EPT >> LR;
-                    if constexpr (Jc > 1) __syncthreads();
-#pragma unroll
-                    for (int i = 0; i < NBp; ++i)
-#pragma unroll
-                        for (int r = 0; r < (1 << LRp); ++r) {
-                            const int e = i * (1 << LRp) + r;
-                            lds4[C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r)) * PH + q] = pack2(v0[e], v1[e]);
-                        }
-                    __syncthreads();
-#pragma unroll
-                    for (int i = 0; i < NB; ++i)
-#pragma unroll
-                        for (int r = 0; r < (1 << LR); ++r) {
-                            const int e = i * (1 << LR) + r;
-                            const float4 w = lds4[C::ppos(g + G * i + r * (N >> LR)) * PH + q];
-                            v0[e] = mk(w.x, w.y);
-                            v1[e] = mk(w.z, w.w);
-                        }
-                }
-                radix_pass<L, Jc, G, RM>(v0, g, tw);
-                radix_pass<L, Jc, G, RM>(v1, g, tw);
-            }(),
-            ...);
-    }(std::make_integer_sequence<int, M>{});
-
-    // last pass outputs: butterfly b = g + G*i, element r at out_pos; positions >= n_out pruned
-    if (active) {
-        float4* __restrict__ dst = reinterpret_cast<float4*>(
-            reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base);
-        const uint32_t sA = (uint32_t)a.out_sA >> 1;
-        constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
-        const int nout = a.n_out;
-#pragma unroll
-        for (int i = 0; i < NBl; ++i)
-#pragma unroll
-            for (int r = 0; r < (1 << LRl); ++r) {
-                const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
-                const int e = i * (1 << LRl) + r;
-                if (pos < nout) dst[q + (uint32_t)pos * sA] = pack2(v0[e], v1[e]);
-            }
     }
 }
 

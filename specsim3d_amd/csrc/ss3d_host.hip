@@ -56,6 +56,7 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 constexpr double kPi = 3.14159265358979323846;  // f90:91
+constexpr int64_t kFillBlocks = 2048;  // FP32 fill passes: fewest blocks per launch before members share blocks
 
 // ---------------------------------------------------------------- host model arithmetic
 // GSLIB scaled rotation, f90:107-139 (row-major rot, axes).
@@ -314,7 +315,8 @@ int build_rng(ss3d_spectrum* sp, int igauss) {
     if (t.ready) return SS3D_OK;
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const uint64_t g = igauss ? 2 : 1;
-    std::vector<uint32_t> tx(npx), rowf((size_t)npy * npz, 0u);
+    // tx has at least 2 entries: the FP32 fill reads the factor of column pairs (c0, c0+1)
+    std::vector<uint32_t> tx(std::max(npx, 2), 0u), rowf((size_t)npy * npz, 0u);
     const uint32_t ax = pm_pow(kPmA, g);
     tx[0] = 1;
     for (int i = 1; i < npx; ++i) tx[i] = pm_mul(tx[i - 1], ax);
@@ -388,13 +390,17 @@ struct Pass {
 
 // Bytes a pass must move through HBM per batch member: the operands it reads (complex input,
 // or the amplitude it fills from) and the results it writes.
-double pass_bytes(const Pass& p) {
+// Algorithmic HBM bytes of one launch over nbatch members: operands read once per block group
+// (`groups` = nbatch, except fill passes whose blocks serve several members), outputs written
+// once per member.
+double pass_bytes(const Pass& p, int nbatch, int groups) {
     const PassArgs& a = p.a;
     const double ec = p.prec == 64 ? 16.0 : 8.0, er = ec / 2;
     const double n = double(int64_t(1) << a.logn);
     const double cols = a.axis == 0 ? double(a.nrows) : double(a.n0) * a.nthird;
     if (p.fast == 2 || p.fast == 3)  // C2R x pass: half rows (npx/2 complex) in, real cropped rows out
-        return (p.in == IN_LOADH || p.fast == 3 ? cols * std::max(1, a.npx / 2) * ec : a.hx * 4.0) + cols * a.n_out * er;
+        return ((p.in == IN_LOADH || p.fast == 3 ? cols * std::max(1, a.npx / 2) * ec : a.hx * 4.0) +
+                cols * a.n_out * er) * nbatch;
     double in = 0, out = 0;
     switch (p.in) {
         case IN_LOAD: in = cols * n * ec; break;
@@ -410,7 +416,7 @@ double pass_bytes(const Pass& p) {
         case OUT_SPEC: out = double(a.npx) * a.npy * a.npz * 8.0 + double(a.hx) * a.npy * a.npz * 4.0; break;
         default: break;
     }
-    return in + out;
+    return in * groups + out * nbatch;
 }
 
 hipEvent_t take_event(ss3d_context* c) {
@@ -438,6 +444,12 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     }
     p.a.nblocks = p.nblocks;
     if (p.nblocks <= 0) return fail(SS3D_EINVAL, "empty pass");
+    // FP32 fill passes: a block serves bpb batch members, as many as keep >= kFillBlocks blocks
+    p.a.nbatch = nbatch;
+    p.a.bpb = 1;
+    if (p.fast == 1 && p.in == IN_FILL32H)
+        p.a.bpb = std::max(1, std::min(nbatch, (int)((int64_t)p.nblocks * nbatch / kFillBlocks)));
+    const int groups = (nbatch + p.a.bpb - 1) / p.a.bpb;
     if (p.out == OUT_SPEC || p.out == OUT_REAL) {
         SS3D_TRY(partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), strm));
         p.a.partial = partial.as<double>();
@@ -451,7 +463,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     }
     hipError_t e;
     if (p.fast == 1)
-        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.nblocks, nbatch, p.a, strm);
+        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.nblocks, groups, p.a, strm);
     else if (p.fast == 2)
         e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, strm);
     else if (p.fast == 3)
@@ -462,7 +474,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
                                      p.sh.nt, hipGetErrorString(e));
     if (c->profiling) {
         HIP_TRY(hipEventRecord(e1, strm));
-        c->prof.push_back({p.kind, e0, e1, pass_bytes(p) * nbatch});
+        c->prof.push_back({p.kind, e0, e1, pass_bytes(p, nbatch, groups)});
     }
     return SS3D_OK;
 }

@@ -62,6 +62,8 @@ struct PassArgs {
     float* amp32h_out;
     double* partial;  // per block: statistics (3 doubles) or spectrum diagnostics (4 doubles)
     int nblocks;      // blocks per batch member
+    int nbatch;       // batch members of the launch
+    int bpb;          // FP32 fill passes: batch members per block (grid y = ceil(nbatch / bpb))
 };
 
 enum InMode { IN_COV = 0, IN_FILL64 = 1, IN_LOAD = 2, IN_FILL32H = 3, IN_LOADH = 4, IN_FILLH = 5 };
