@@ -214,8 +214,8 @@ extern __shared__ __attribute__((aligned(16))) unsigned char fast_smem[];
 
 // ------------------------------------------------------------ one Stockham pass in registers
 // Thread owns butterflies b = g + G*i, i < EPT/R; v[i*R + r] is element r of butterfly i.
-template <int L, int J, int G, int RM>
-__device__ __forceinline__ void radix_pass(f2* v, int g, const f2* __restrict__ tw) {
+template <int L, int J, int G, int RM, typename TW>
+__device__ __forceinline__ void radix_pass(f2* v, int g, TW tw) {
     constexpr int LR = rlog(L, J, RM), R = 1 << LR, NS = 1 << nslog(L, J, RM);
     constexpr int NB = (1 << eptlog(L, RM)) / R;
 #pragma unroll
@@ -226,11 +226,21 @@ __device__ __forceinline__ void radix_pass(f2* v, int g, const f2* __restrict__ 
             // W_{NS R}^{r k} = W_n^{r k n/(NS R)}
             constexpr int SH = L - nslog(L, J, RM) - LR;
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[i * R + r] = mul(__ldg(tw + ((r * k) << SH)), v[i * R + r]);
+            for (int r = 1; r < R; ++r) v[i * R + r] = mul(tw((r * k) << SH), v[i * R + r]);
         }
         idft<LR>(v + i * R);
     }
 }
+
+// twiddle table W_n^m read from HBM (cached) or from an LDS copy
+struct TwGlobal {
+    const f2* __restrict__ p;
+    __device__ __forceinline__ f2 operator()(int m) const { return __ldg(p + m); }
+};
+struct TwLds {
+    const f2* p;
+    __device__ __forceinline__ f2 operator()(int m) const { return p[m]; }
+};
 
 // output position of element r of butterfly b in pass J
 template <int L, int J, int RM>
@@ -257,6 +267,10 @@ struct StridedCfg {
     static constexpr bool PADPOS = P < 16;
     static constexpr int ppos(int pos) { return PADPOS ? pos + (pos >> EL) : pos; }
     static constexpr size_t lds_elems() { return (size_t)(N + (PADPOS ? (N >> EL) : 0)) * P; }
+    // lengths up to 2^11: the W_n table (n entries) is copied to LDS behind the exchange buffer,
+    // so that no pass waits on vector-memory loads (which complete in order with the stores)
+    static constexpr bool TWL = L <= 11;
+    static constexpr size_t lds_bytes() { return lds_elems() * sizeof(f2) + (TWL ? (size_t)N * sizeof(f2) : 0); }
 };
 
 __device__ __forceinline__ float4 pack2(f2 a, f2 b) { return make_float4(a.x, a.y, b.x, b.y); }
@@ -273,8 +287,17 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
     static_assert(M > 0, "strided passes have length >= 2");
     float4* lds4 = reinterpret_cast<float4*>(fast_smem);
     const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);
+    f2* twl = reinterpret_cast<f2*>(fast_smem + C::lds_elems() * sizeof(f2));
     const int tid = threadIdx.x;
     const int q = tid % PH, g = tid / PH;
+    if constexpr (C::TWL && M > 1) {
+        // first read after the barrier of the first exchange
+        for (int m = tid; m < N; m += C::NT) twl[m] = __ldg(tw + m);
+    }
+    auto twid = [&] {
+        if constexpr (C::TWL) return TwLds{twl};
+        else return TwGlobal{tw};
+    }();
     const int tile = blockIdx.x;
     const int i0base = (tile % a.ntiles0) * P;
     const int third = tile / a.ntiles0;
@@ -311,8 +334,8 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
                                 v1[e] = mk(w.z, w.w);
                             }
                     }
-                    radix_pass<L, Jc, G, RM>(v0, g, tw);
-                    radix_pass<L, Jc, G, RM>(v1, g, tw);
+                    radix_pass<L, Jc, G, RM>(v0, g, twid);
+                    radix_pass<L, Jc, G, RM>(v1, g, twid);
                 }(),
                 ...);
         }(std::make_integer_sequence<int, M>{});
@@ -357,6 +380,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
         // site is the partner (conjugate).  A site is self-conjugate only in column 0 (kx = 0,
         // and the Nyquist column it carries) on a self row (row == partner row): selfm bit r.
         uint32_t w0[EPT], w1[EPT];
+        float2 ampv[EPT];
         uint32_t selfm = 0;
         const bool col0 = active && c0 == 0, pack = col0 && a.npx >= 2;
         const int kxn = a.npx >> 1;
@@ -377,6 +401,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
                 const int pos = g + r * G;
                 const Site sv = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
                 const uint32_t rf = __ldg(a.rowf + sv.site);
+                ampv[r] = __ldg(reinterpret_cast<const float2*>(abase + r * astep));
                 // the site row is shared by every column; x order decides only inside self rows
                 const bool ja = sv.gt || (sv.eq && kxa > pa);
                 const bool jb = sv.gt || (sv.eq && kxb > pb);
@@ -387,28 +412,30 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
         }
         for (int b = b0; b < b1; ++b) {
             if (b > b0) __syncthreads();  // LDS reads of the previous member are done
+            // realization base: a uniform read through the scalar cache (written by an earlier
+            // kernel, never by this one), so that no vector-memory wait (which would also wait
+            // for the stores of the previous member) is needed
+            const uint32_t rb = ((const __attribute__((address_space(4))) uint32_t*)a.rbase)[b];
             if (active) {
-                const uint32_t rb = __ldg(a.rbase + b);
-                // keep the site values and the amplitude base opaque so that nothing derived from
-                // them is hoisted out of the member loop (registers); the amplitudes are re-read (L2)
-                const float* ab = abase;
-                asm volatile("" : "+v"(ab));
 #pragma unroll
                 for (int r = 0; r < EPT; ++r) {
-                    asm volatile("" : "+v"(w0[r]), "+v"(w1[r]));
-                    const float2 am = __ldg(reinterpret_cast<const float2*>(ab + r * astep));
-                    v0[r] = coef(am.x, pm_mul(w0[r] & 0x7fffffffu, rb), a.igauss, w0[r] >> 31, false);
-                    v1[r] = coef(am.y, pm_mul(w1[r] & 0x7fffffffu, rb), a.igauss, w1[r] >> 31, false);
+                    // keep the site values opaque so that nothing derived from them is hoisted
+                    // out of the member loop (registers)
+                    asm volatile("" : "+v"(w0[r]), "+v"(w1[r]), "+v"(ampv[r].x), "+v"(ampv[r].y));
+                    v0[r] = coef(ampv[r].x, pm_mul(w0[r] & 0x7fffffffu, rb), a.igauss, w0[r] >> 31, false);
+                    v1[r] = coef(ampv[r].y, pm_mul(w1[r] & 0x7fffffffu, rb), a.igauss, w1[r] >> 31, false);
                 }
                 if (col0) {  // kx = 0: self-conjugate sites; the Nyquist column (kx = npx/2, its own partner)
+                    int zero = 0;  // opaque: the addresses below are not kept across the loop
+                    asm volatile("" : "+v"(zero));
+                    const float* __restrict__ ab = abase + zero;
 #pragma unroll
                     for (int r = 0; r < EPT; ++r) {
                         const uint32_t st = pm_mul(w0[r] & 0x7fffffffu, rb);  // column factor of kx = 0 is 1
                         const bool cj = w0[r] >> 31, sf = (selfm >> r) & 1u;
-                        const float* ap = ab + r * astep;
-                        if (sf) v0[r] = coef(__ldg(ap), st, a.igauss, false, true);
+                        if (sf) v0[r] = coef(ampv[r].x, st, a.igauss, false, true);
                         if (pack) {
-                            const f2 zn = coef(__ldg(ap + kxn), pm_mul(st, txn), a.igauss, cj, sf);
+                            const f2 zn = coef(__ldg(ab + r * astep + kxn), pm_mul(st, txn), a.igauss, cj, sf);
                             v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
                         }
                     }
@@ -513,7 +540,7 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
                         for (int r = 0; r < (1 << LR); ++r)
                             v[i * (1 << LR) + r] = lds[q * C::ROW + C::ppos(g + G * i + r * (N >> LR))];
                 }
-                radix_pass<L, Jc, G, RM>(v, g, tw);
+                radix_pass<L, Jc, G, RM>(v, g, TwGlobal{tw});
             }(),
             ...);
     }(std::make_integer_sequence<int, M>{});

@@ -24,7 +24,7 @@ template <int L, int IN, int AXIS, int RM>
 hipError_t strided_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using C = fast::StridedCfg<L, strided_P(L), RM>;
     static bool attr = false;
-    return launch_k(fast::strided_kernel<L, strided_P(L), IN, AXIS, RM>, C::lds_elems() * sizeof(float2), C::NT,
+    return launch_k(fast::strided_kernel<L, strided_P(L), IN, AXIS, RM>, C::lds_bytes(), C::NT,
                     nblocks, nbatch, a, s, attr);
 }
 
