@@ -130,10 +130,11 @@ __device__ __forceinline__ uint32_t pm_mul(uint32_t a, uint32_t b) {
 // +-amp), igauss=1 amp*(g1 + i g2)/sqrt2 (self-conjugate: amp*g1), Box-Muller on (u1, u2);
 // conj for the partner of a drawing site.  Branch-free (selects only).  Hardware sin/cos take
 // revolutions: v_sin_f32(x) = sin(2 pi x); v_log_f32 is log2.
-__device__ __forceinline__ f2 coef(float amp, uint32_t st, int igauss, bool conj, bool selfc) {
+template <int IG>
+__device__ __forceinline__ f2 coef_t(float amp, uint32_t st, bool conj, bool selfc) {
     const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
     float re, im;
-    if (!igauss) {  // uniform branch (kernel argument)
+    if constexpr (IG == 0) {
         const float u = (float)st * inv_m;
         const float c = __builtin_amdgcn_cosf(u), s = __builtin_amdgcn_sinf(u);
         const float sgn = (st <= 1073741823u) ? amp : -amp;
@@ -149,6 +150,9 @@ __device__ __forceinline__ f2 coef(float amp, uint32_t st, int igauss, bool conj
         im = selfc ? 0.0f : a * __builtin_amdgcn_sinf(u2);
     }
     return mk(re, conj ? -im : im);
+}
+__device__ __forceinline__ f2 coef(float amp, uint32_t st, int igauss, bool conj, bool selfc) {
+    return igauss ? coef_t<1>(amp, st, conj, selfc) : coef_t<0>(amp, st, conj, selfc);
 }
 
 // Fill of a half-spectrum column kx (kx <= npx/2), following the walk of
@@ -275,6 +279,24 @@ struct StridedCfg {
 
 __device__ __forceinline__ float4 pack2(f2 a, f2 b) { return make_float4(a.x, a.y, b.x, b.y); }
 
+// Raw buffer access (offsets in bytes from a uniform base).  An offset >= kOob is out of range:
+// a load returns zeros and a store is dropped, without touching memory -- so masked lanes and
+// members past the end need no branch, and every wave issues the same memory instructions
+// (which keeps the compiler's vector-memory counter waits exact across the member loop).
+using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+constexpr uint32_t kOob = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ float4 bload16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
+}
+__device__ __forceinline__ void bstore16(float4 v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const u32x4 w = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+    __builtin_amdgcn_raw_buffer_store_b128(w, rs, off, 0, 0);
+}
+
 // Fill mode (IN_FILL32H): a block serves realizations b0 .. b1-1 of its tile (a.bpb per block).
 // What depends on the site and not on the realization -- the amplitude, the product of the
 // site's row factor and column factor, the conjugation and self-conjugate flags -- is read and
@@ -339,12 +361,23 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
                 }(),
                 ...);
         }(std::make_integer_sequence<int, M>{});
-        if (active) {
-            float4* __restrict__ dst = reinterpret_cast<float4*>(
-                reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base);
-            const uint32_t sA = (uint32_t)a.out_sA >> 1;
-            constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
-            const int nout = a.n_out;
+        constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
+        const uint32_t sA = (uint32_t)a.out_sA >> 1;  // in float4 units
+        const int nout = a.n_out;
+        const f2* obase = reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base;
+        if constexpr (IN == IN_LOAD) {  // branch-free: pruned and inactive lanes store out of range
+            const __amdgpu_buffer_rsrc_t rs = rsrc(obase);
+#pragma unroll
+            for (int i = 0; i < NBl; ++i)
+#pragma unroll
+                for (int r = 0; r < (1 << LRl); ++r) {
+                    const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
+                    const int e = i * (1 << LRl) + r;
+                    const uint32_t off = (active && pos < nout) ? (q + (uint32_t)pos * sA) * 16u : kOob;
+                    bstore16(pack2(v0[e], v1[e]), rs, off);
+                }
+        } else if (active) {
+            float4* __restrict__ dst = reinterpret_cast<float4*>(const_cast<f2*>(obase));
 #pragma unroll
             for (int i = 0; i < NBl; ++i)
 #pragma unroll
@@ -357,23 +390,45 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
     };
 
     if constexpr (IN == IN_LOAD) {  // pass-0 operands at positions g + r*G
-        const int b = blockIdx.y;
-        const float4* __restrict__ src = reinterpret_cast<const float4*>(
-            reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride + (int64_t)third * a.in_sT + i0base);
+        // members b0 .. b1-1 of this tile; the operands of member b+1 are loaded while member b
+        // is transformed and stored
+        const int b0 = blockIdx.y * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
         const uint32_t sA = (uint32_t)a.in_sA >> 1;  // in float4 units
         const uint32_t o0 = q + (uint32_t)g * sA, step = (uint32_t)G * sA;
-        if (active) {
+        float4 nxt[EPT];
+        auto load = [&](int b) {  // b >= b1: zeros, no memory access
+            const int bb = min(b, b1 - 1);
+            const __amdgpu_buffer_rsrc_t rs = rsrc(reinterpret_cast<const f2*>(a.in) + (int64_t)bb * a.in_bstride +
+                                                   (int64_t)third * a.in_sT + i0base);
+            const bool ok = active && b < b1;
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) nxt[r] = bload16(rs, ok ? (o0 + r * step) * 16u : kOob);
+        };
+        auto consume = [&] {
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {
-                const float4 w = src[o0 + r * step];
-                v0[r] = mk(w.x, w.y);
-                v1[r] = mk(w.z, w.w);
+                v0[r] = mk(nxt[r].x, nxt[r].y);
+                v1[r] = mk(nxt[r].z, nxt[r].w);
             }
-        } else {
+        };
+        load(b0);
+        {
+            // as many out-of-range (dropped) stores as a member issues: the loop is entered with
+            // the same vector-memory counter history as from its back edge, so the wait for the
+            // prefetched operands does not also wait for the previous member's stores
+            const __amdgpu_buffer_rsrc_t rs = rsrc(a.out);
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
+            for (int e = 0; e < EPT; ++e) bstore16(make_float4(0.f, 0.f, 0.f, 0.f), rs, kOob + 16u * e);
         }
-        transform_store(b, g, q);
+        consume();
+        for (int b = b0; b < b1; ++b) {
+            load(b + 1);  // in flight while member b is transformed and stored
+            int gg = g, qq = q;
+            asm volatile("" : "+v"(gg), "+v"(qq));
+            transform_store(b, gg, qq);
+            consume();
+            __syncthreads();  // LDS reads of member b are done before the exchange of b+1
+        }
     } else {  // IN_FILL32H: AXIS 2: (ky, kz) = (third, pos), AXIS 1: (pos, third)
         const int b0 = blockIdx.y * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
         // Per element and column: row factor x column factor of the site, bit 31 set when the

@@ -56,7 +56,7 @@ int fail(int code, const char* fmt, ...) {
     } while (0)
 
 constexpr double kPi = 3.14159265358979323846;  // f90:91
-constexpr int64_t kFillBlocks = 2048;  // FP32 fill passes: fewest blocks per launch before members share blocks
+constexpr int64_t kFillBlocks = 2048;  // FP32 strided passes: fewest blocks per launch before members share blocks
 
 // ---------------------------------------------------------------- host model arithmetic
 // GSLIB scaled rotation, f90:107-139 (row-major rot, axes).
@@ -390,9 +390,9 @@ struct Pass {
 
 // Bytes a pass must move through HBM per batch member: the operands it reads (complex input,
 // or the amplitude it fills from) and the results it writes.
-// Algorithmic HBM bytes of one launch over nbatch members: operands read once per block group
-// (`groups` = nbatch, except fill passes whose blocks serve several members), outputs written
-// once per member.
+// Algorithmic HBM bytes of one launch over nbatch members: outputs written once per member,
+// operands read once per member, except the amplitudes of fill passes, read once per block group
+// (`groups` = ceil(nbatch / members per block)).
 double pass_bytes(const Pass& p, int nbatch, int groups) {
     const PassArgs& a = p.a;
     const double ec = p.prec == 64 ? 16.0 : 8.0, er = ec / 2;
@@ -416,7 +416,7 @@ double pass_bytes(const Pass& p, int nbatch, int groups) {
         case OUT_SPEC: out = double(a.npx) * a.npy * a.npz * 8.0 + double(a.hx) * a.npy * a.npz * 4.0; break;
         default: break;
     }
-    return in * groups + out * nbatch;
+    return in * (p.in == IN_FILL32H ? groups : nbatch) + out * nbatch;
 }
 
 hipEvent_t take_event(ss3d_context* c) {
@@ -444,12 +444,18 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     }
     p.a.nblocks = p.nblocks;
     if (p.nblocks <= 0) return fail(SS3D_EINVAL, "empty pass");
-    // FP32 fill passes: a block serves bpb batch members, as many as keep >= kFillBlocks blocks
+    // FP32 strided passes: a block serves bpb batch members, as many as keep >= kFillBlocks blocks
     p.a.nbatch = nbatch;
     p.a.bpb = 1;
-    if (p.fast == 1 && p.in == IN_FILL32H)
+    if (p.fast == 1)
         p.a.bpb = std::max(1, std::min(nbatch, (int)((int64_t)p.nblocks * nbatch / kFillBlocks)));
     const int groups = (nbatch + p.a.bpb - 1) / p.a.bpb;
+    if (p.fast == 1 && p.in == IN_LOAD) {  // buffer addressing: byte offsets inside a tile below 2^31
+        const int64_t span_in = ((int64_t(1) << p.a.logn) * p.a.in_sA + P) * 8;
+        const int64_t span_out = ((int64_t)p.a.n_out * p.a.out_sA + P) * 8;
+        if (span_in > 0x7fffffffLL || span_out > 0x7fffffffLL)
+            return fail(SS3D_ETOOBIG, "FP32 pass tile spans more than 2 GiB");
+    }
     if (p.out == OUT_SPEC || p.out == OUT_REAL) {
         SS3D_TRY(partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), strm));
         p.a.partial = partial.as<double>();
