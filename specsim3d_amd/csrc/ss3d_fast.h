@@ -142,7 +142,8 @@ __device__ __forceinline__ f2 coef_t(float amp, uint32_t st, bool conj, bool sel
         im = selfc ? 0.0f : amp * s;
     } else {
         const float u1 = (float)st * inv_m;
-        const float u2 = (float)pm_mul(st, kPmA) * inv_m;
+        // u2 = (st a mod M)/M = frac(st a/M) (to 4e-12): only its sin/cos (period 1) are used
+        const float u2 = (float)__builtin_amdgcn_fract((double)st * (16807.0 / 2147483647.0));
         // sqrt(-2 ln u1) = sqrt(-2 ln2 log2 u1)
         const float rr = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
         const float a = amp * rr * (selfc ? 1.0f : 0.70710678118654752440f);
