@@ -131,10 +131,12 @@ __device__ __forceinline__ uint32_t pm_mul(uint32_t a, uint32_t b) {
 // conj for the partner of a drawing site.  Branch-free (selects only).  Hardware sin/cos take
 // revolutions: v_sin_f32(x) = sin(2 pi x); v_log_f32 is log2.
 template <int IG>
-__device__ __forceinline__ f2 coef_t(float amp, uint32_t st, bool conj, bool selfc) {
+__device__ __forceinline__ f2 coef_t(float amps, uint32_t st, bool conj, bool selfc) {
+    // amps = amplitude * sqrt(ln 2) (the FP32 half-spectrum table)
     const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
     float re, im;
     if constexpr (IG == 0) {
+        const float amp = amps * 1.2011224087864498f;  // 1/sqrt(ln 2)
         const float u = (float)st * inv_m;
         const float c = __builtin_amdgcn_cosf(u), s = __builtin_amdgcn_sinf(u);
         const float sgn = (st <= 1073741823u) ? amp : -amp;
@@ -144,9 +146,8 @@ __device__ __forceinline__ f2 coef_t(float amp, uint32_t st, bool conj, bool sel
         const float u1 = (float)st * inv_m;
         // u2 = (st a mod M)/M = frac(st a/M) (to 4e-12): only its sin/cos (period 1) are used
         const float u2 = (float)__builtin_amdgcn_fract((double)st * (16807.0 / 2147483647.0));
-        // sqrt(-2 ln u1) = sqrt(-2 ln2 log2 u1)
-        const float rr = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
-        const float a = amp * rr * (selfc ? 1.0f : 0.70710678118654752440f);
+        // amp sqrt(-2 ln u1) / sqrt(2) = amps sqrt(-log2 u1); self-conjugate sites: times sqrt(2)
+        const float a = amps * __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1)) * (selfc ? 1.41421356237309505f : 1.0f);
         re = a * __builtin_amdgcn_cosf(u2);
         im = selfc ? 0.0f : a * __builtin_amdgcn_sinf(u2);
     }

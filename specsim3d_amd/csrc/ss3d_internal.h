@@ -14,6 +14,9 @@ constexpr uint32_t kPmM = 2147483647u;  // Park-Miller modulus 2^31-1 (f90:193)
 constexpr uint32_t kPmA = 16807u;       // Park-Miller multiplier (f90:193)
 constexpr int kEPT = 16;           // complex elements per thread in every FFT pass
 constexpr int kPadShift = 4;       // LDS padding: one slot after every 16 complex elements
+// The FP32 half-spectrum amplitudes are stored times sqrt(ln 2): amp sqrt(-2 ln u1)/sqrt(2) =
+// (amp sqrt(ln 2)) sqrt(-log2 u1), one multiplication fewer per Gaussian coefficient.
+constexpr double kAmp32Scale = 0.83255461115769775635;
 
 // One covariance structure as the kernels use it (setrot already applied, f90:107-139).
 struct StructDev {
@@ -52,7 +55,7 @@ struct PassArgs {
     StructDev st[kMaxStruct];
     // coefficient fill (realization passes)
     const double* amp64;     // npx*npy*npz
-    const float* amp32h;     // hxa*npy*npz (columns kx <= npx/2 used)
+    const float* amp32h;     // hxa*npy*npz (columns kx <= npx/2 used), amplitude * kAmp32Scale
     const uint32_t* rowf;    // per drawing row: a^(g*(draw-point index of its first point))
     const uint32_t* txp;     // a^(g*i), i < npx
     const uint32_t* rbase;   // per realization: state before its first draw, times a

@@ -133,40 +133,6 @@ __device__ __forceinline__ cpx<double> coef64(double amp, uint32_t st, int igaus
     return c;
 }
 
-// FP32 coefficient from the same integer draw (phases through sinpi/cospi of 2u).
-__device__ __forceinline__ cpx<float> coef32(float amp, uint32_t st, int igauss, const DrawSite& d) {
-    const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
-    const float s05 = 0.70710678118654752440f;
-    cpx<float> c;
-    if (!igauss) {
-        if (d.selfc) {
-            c.x = (st <= 1073741823u) ? amp : -amp;
-            c.y = 0.0f;
-        } else {
-            float u = (float)st * inv_m;
-            float sn, cs;
-            sincospif(2.0f * u, &sn, &cs);
-            c.x = amp * cs;
-            c.y = d.conj ? -(amp * sn) : amp * sn;
-        }
-    } else {
-        float u1 = (float)st * inv_m;
-        float u2 = (float)pm_mul(st, kPmA) * inv_m;
-        float rr = sqrtf(-2.0f * logf(u1));
-        float sn, cs;
-        sincospif(2.0f * u2, &sn, &cs);
-        float g1 = rr * cs, g2 = rr * sn;
-        if (d.selfc) {
-            c.x = amp * g1;
-            c.y = 0.0f;
-        } else {
-            c.x = amp * g1 * s05;
-            c.y = d.conj ? -(amp * g2 * s05) : amp * g2 * s05;
-        }
-    }
-    return c;
-}
-
 // Covariance at a lag, f90:153-178, in the reference's operation order.
 __device__ __forceinline__ double cov64(const PassArgs& a, double hx, double hy, double hz) {
     double c = 0.0;
@@ -333,7 +299,7 @@ __global__ __launch_bounds__(NT) void pass_kernel(PassArgs a) {
         row0 = tile * P;
     }
     uint32_t rb = 0;
-    if constexpr (IN == IN_FILL64 || IN == IN_FILL32H || IN == IN_FILLH) rb = __ldg(a.rbase + b);
+    if constexpr (IN == IN_FILL64) rb = __ldg(a.rbase + b);
 
     // ---- input stage
 #pragma unroll 4
@@ -354,13 +320,6 @@ __global__ __launch_bounds__(NT) void pass_kernel(PassArgs a) {
                 if constexpr (IN == IN_LOAD) {
                     const cpx<R>* src = reinterpret_cast<const cpx<R>*>(a.in) + (int64_t)b * a.in_bstride;
                     v = src[i0 + pos * a.in_sA + third * a.in_sT];
-                } else if constexpr (IN == IN_FILL32H) {
-                    // half spectrum column kx = i0; axis 2: (ky, kz) = (third, pos); axis 1: (pos, third)
-                    const int ky = a.axis == 2 ? third : pos;
-                    const int kz = a.axis == 2 ? pos : third;
-                    const float amp = a.amp32h[i0 + (int64_t)a.hx * (ky + (int64_t)a.npy * kz)];
-                    const DrawSite d = classify(i0, ky, kz, a.npx, a.npy, a.npz);
-                    if constexpr (sizeof(R) == 4) v = coef32(amp, site_state(a, rb, d), a.igauss, d);
                 }
             }
         } else {
@@ -381,19 +340,6 @@ __global__ __launch_bounds__(NT) void pass_kernel(PassArgs a) {
                     const double amp = a.amp64[pos + (int64_t)a.npx * row];
                     const DrawSite d = classify(pos, j, k, a.npx, a.npy, a.npz);
                     if constexpr (sizeof(R) == 8) v = coef64(amp, site_state(a, rb, d), a.igauss, d);
-                } else if constexpr (IN == IN_LOADH || IN == IN_FILLH) {
-                    // x line of a Hermitian 3D spectrum: entries above npx/2 mirror the half row
-                    const bool mir = pos > (a.npx >> 1);
-                    const int kk = mir ? a.npx - pos : pos;
-                    if constexpr (IN == IN_LOADH) {
-                        const cpx<R>* src = reinterpret_cast<const cpx<R>*>(a.in) + (int64_t)b * a.in_bstride;
-                        v = src[(int64_t)row * a.in_row + kk];
-                    } else {
-                        const float amp = a.amp32h[kk];
-                        const DrawSite d = classify(kk, 0, 0, a.npx, a.npy, a.npz);
-                        if constexpr (sizeof(R) == 4) v = coef32(amp, site_state(a, rb, d), a.igauss, d);
-                    }
-                    if (mir) v.y = -v.y;
                 }
             }
         }
@@ -453,7 +399,8 @@ __global__ __launch_bounds__(NT) void pass_kernel(PassArgs a) {
             }
             const int64_t row = j + (int64_t)a.npy * k;
             if (a.amp64_out) a.amp64_out[i + (int64_t)a.npx * row] = amp;
-            if (a.amp32h_out && i < a.hx) a.amp32h_out[i + (int64_t)a.hxa * row] = (float)amp;
+            // FP32 half spectrum: amplitude times sqrt(ln 2), the factor of the FP32 Gaussian law
+            if (a.amp32h_out && i < a.hx) a.amp32h_out[i + (int64_t)a.hxa * row] = (float)(amp * kAmp32Scale);
         }
         const int ops[4] = {1, 1, 0, 0};
         block_reduce_write<NT, 4>(acc, ops, a.partial + 4 * ((int64_t)b * a.nblocks + blockIdx.x));
