@@ -609,6 +609,7 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
         const uint32_t base = (uint32_t)q * (uint32_t)a.out_row;
         constexpr int LRl = M ? rlog(L, M > 0 ? M - 1 : 0, RM) : 0, NBl = EPT >> LRl;
         const int nx = a.n_out;
+        const bool even = ((a.out_row | a.out_bstride) & 1) == 0;  // 8-byte aligned pairs: one float2 store
 #pragma unroll
         for (int i = 0; i < NBl; ++i)
 #pragma unroll
@@ -617,8 +618,12 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
                 if constexpr (M > 0) m = out_pos<L, M - 1, RM>(g + G * i, r);
                 const f2 z = v[i * (1 << LRl) + r];
                 if (2 * m + 1 < nx) {
-                    dst[base + 2 * m] = z.x;
-                    dst[base + 2 * m + 1] = z.y;
+                    if (even) {
+                        *reinterpret_cast<float2*>(dst + base + 2 * m) = make_float2(z.x, z.y);
+                    } else {
+                        dst[base + 2 * m] = z.x;
+                        dst[base + 2 * m + 1] = z.y;
+                    }
                     s1 += (double)z.x + (double)z.y;
                     s2 += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
                 } else if (2 * m < nx) {
