@@ -609,7 +609,8 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
         const uint32_t base = (uint32_t)q * (uint32_t)a.out_row;
         constexpr int LRl = M ? rlog(L, M > 0 ? M - 1 : 0, RM) : 0, NBl = EPT >> LRl;
         const int nx = a.n_out;
-        const bool even = ((a.out_row | a.out_bstride) & 1) == 0;  // 8-byte aligned pairs: one float2 store
+        // 8-byte aligned pairs: one float2 store
+        const bool even = ((a.out_row | a.out_bstride) & 1) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 7) == 0;
 #pragma unroll
         for (int i = 0; i < NBl; ++i)
 #pragma unroll
