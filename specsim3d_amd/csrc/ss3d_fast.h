@@ -322,7 +322,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
         if constexpr (C::TWL) return TwLds{twl};
         else return TwGlobal{tw};
     }();
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int i0base = (tile % a.ntiles0) * P;
     const int third = tile / a.ntiles0;
     const int c0 = i0base + 2 * q;

@@ -69,6 +69,15 @@ struct PassArgs {
     int bpb;          // FP32 fill passes: batch members per block (grid y = ceil(nbatch / bpb))
 };
 
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of tiles so
+// that neighbouring tiles (which share cache lines and table entries) share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    constexpr int X = 8;
+    const int per = nb / X, rem = nb % X;
+    const int xcd = bid % X, loc = bid / X;
+    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
+}
+
 enum InMode { IN_COV = 0, IN_FILL64 = 1, IN_LOAD = 2, IN_FILL32H = 3, IN_LOADH = 4, IN_FILLH = 5 };
 enum OutMode { OUT_STORE = 0, OUT_SPEC = 1, OUT_REAL = 2 };
 

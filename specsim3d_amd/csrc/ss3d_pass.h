@@ -49,15 +49,6 @@ __device__ __forceinline__ cpx<R> cmul(cpx<R> w, cpx<R> x) {
 
 __device__ __forceinline__ int lds_slot(int idx) { return idx + (idx >> kPadShift); }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of tiles so
-// neighbouring tiles share that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nb) {
-    constexpr int X = 8;
-    int per = nb / X, rem = nb % X;
-    int xcd = bid % X, loc = bid / X;
-    return xcd < rem ? xcd * (per + 1) + loc : rem * (per + 1) + (xcd - rem) * per + loc;
-}
-
 // ---------------------------------------------------------------- Park-Miller on the device
 // state' = a*state mod (2^31-1) (f90:193), product folded with 2^31 == 1 (mod 2^31-1).
 __device__ __forceinline__ uint32_t pm_mul(uint32_t a, uint32_t b) {
