@@ -296,7 +296,13 @@ __device__ __forceinline__ float4 bload16(__amdgpu_buffer_rsrc_t rs, uint32_t of
 }
 __device__ __forceinline__ void bstore16(float4 v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
     const u32x4 w = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
-    __builtin_amdgcn_raw_buffer_store_b128(w, rs, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(w, rs, off, 0, 2);  // aux 2: non-temporal (streamed out)
+}
+// non-temporal 16-byte store: pass outputs are read back by the next pass, long after L2 has
+// cycled, so they are not kept in cache
+__device__ __forceinline__ void st_nt16(float4 v, float4* p) {
+    using f4 = float __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store((f4){v.x, v.y, v.z, v.w}, reinterpret_cast<f4*>(p));
 }
 
 // Fill mode (IN_FILL32H): a block serves realizations b0 .. b1-1 of its tile (a.bpb per block).
@@ -386,7 +392,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
                 for (int r = 0; r < (1 << LRl); ++r) {
                     const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
                     const int e = i * (1 << LRl) + r;
-                    if (pos < nout) dst[q + (uint32_t)pos * sA] = pack2(v0[e], v1[e]);
+                    if (pos < nout) st_nt16(pack2(v0[e], v1[e]), dst + q + (uint32_t)pos * sA);
                 }
         }
     };
