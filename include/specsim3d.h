@@ -172,6 +172,23 @@ double ss3d_server_clock(const ss3d_server* s);
 /* Waits for every submitted request, then releases the lanes and the cache. */
 int ss3d_server_destroy(ss3d_server* s);
 
+/* ---------------------------------------------------------------- variograms
+ * Experimental semivariograms, replacing vario_lag (src/specsim3d.f90:405-431) as vario_report
+ * calls it (f90:462-483).  For realization r, direction d with step (sx, sy, sz) and lag
+ * l = 1..nlag_d:
+ *   gam[(r*ndir + d)*nlagmax + l-1] = 0.5 * sum over all node pairs (f(x + l s) - f(x))^2 / npairs,
+ * 0 where no pair exists (and for l > nlag_d); nlagmax = max(1, max_d nlag_d).  field: nreal fields
+ * of nx*ny*nz values, x fastest, FP64 (SS3D_FP64) or float32 (SS3D_FP32), in host memory
+ * (on_device = 0, copied in) or device memory.  dirs: ndir x {sx, sy, sz, nlag}.  Sums are FP64 in
+ * a fixed order, so repeated calls give identical results. */
+int ss3d_variogram(ss3d_context* ctx, const void* field, int on_device, int precision, int nx, int ny, int nz,
+                   int nreal, int ndir, const int32_t* dirs, double* gam);
+/* Number of node pairs at lag (lx, ly, lz) on an nx x ny x nz grid (npair_count, f90:394-402). */
+int64_t ss3d_npairs(int nx, int ny, int nz, int lx, int ly, int lz);
+/* Model covariance C(h) of nested structures plus nugget at lag (hx, hy, hz) (covariance, f90:153-178),
+ * FP64 on the host; the model variogram is C(0) - C(h). */
+double ss3d_covariance(const ss3d_structure* st, int nst, double nugget, double hx, double hy, double hz);
+
 /* ---------------------------------------------------------------- batch graph
  * No counterpart in the reference.  Many FP32 jobs of uneven grids (ss3d_request, host_out
  * ignored), simulated together: creation builds their spectral densities (shared between

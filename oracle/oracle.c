@@ -455,3 +455,28 @@ int or_simulate(const or_structure* st, int nst, double nugget, double dx, doubl
     free(amp);
     return 0;
 }
+
+/* npair_count, f90:394-402: pairs at lag (lx, ly, lz) on an nx x ny x nz grid. */
+int64_t or_npair_count(int nx, int ny, int nz, int lx, int ly, int lz) {
+    if (abs(lx) >= nx || abs(ly) >= ny || abs(lz) >= nz) return 0;
+    return (int64_t)(nx - abs(lx)) * (int64_t)(ny - abs(ly)) * (int64_t)(nz - abs(lz));
+}
+
+/* vario_lag, f90:405-431: the loop runs over the nodes whose partner x + l is inside the grid,
+ * k outermost, accumulating (f(x + l) - f(x))^2 in that order. */
+double or_vario_lag(const double* f, int nx, int ny, int nz, int lx, int ly, int lz) {
+    const int64_t np = or_npair_count(nx, ny, nz, lx, ly, lz);
+    if (np == 0) return 0.0;
+    const int i0 = lx < 0 ? -lx : 0, i1 = lx > 0 ? nx - lx : nx;
+    const int j0 = ly < 0 ? -ly : 0, j1 = ly > 0 ? ny - ly : ny;
+    const int k0 = lz < 0 ? -lz : 0, k1 = lz > 0 ? nz - lz : nz;
+    double acc = 0.0;
+    for (int k = k0; k < k1; ++k)
+        for (int j = j0; j < j1; ++j)
+            for (int i = i0; i < i1; ++i) {
+                const double d = f[(i + lx) + (int64_t)nx * ((j + ly) + (int64_t)ny * (k + lz))] -
+                                 f[i + (int64_t)nx * (j + (int64_t)ny * k)];
+                acc = acc + d * d;
+            }
+    return 0.5 * acc / (double)np;
+}

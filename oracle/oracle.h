@@ -73,6 +73,12 @@ int     or_simulate(const or_structure* st, int nst, double nugget, double dx, d
                     double* res, double* mean, double* var, double* maximag, or_spec_diag* diag,
                     int* np_out /* 3 ints, may be NULL */);
 
+/* Experimental semivariogram of one field (x fastest) for the integer lag vector (lx, ly, lz):
+ * 0.5 * mean over all node pairs of (f(x + l) - f(x))^2, 0 when no pair exists
+ * (npair_count f90:394-402, vario_lag f90:405-431). */
+int64_t or_npair_count(int nx, int ny, int nz, int lx, int ly, int lz);
+double  or_vario_lag(const double* f, int nx, int ny, int nz, int lx, int ly, int lz);
+
 #ifdef __cplusplus
 }
 #endif

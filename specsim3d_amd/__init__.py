@@ -104,6 +104,11 @@ SIGNATURES = {
     "ss3d_batch_output": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "ss3d_batch_cells": (ctypes.c_int64, [_vp]),
     "ss3d_batch_destroy": (ctypes.c_int, [_vp]),
+    "ss3d_variogram": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "ss3d_npairs": (ctypes.c_int64, [ctypes.c_int] * 6),
+    "ss3d_covariance": (ctypes.c_double, [ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double]),
     "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
 }
 
@@ -228,6 +233,30 @@ class Engine:
 
     def synchronize(self):
         check(lib().ss3d_context_synchronize(self.handle))
+
+    def variogram(self, field, dirs, device_shape=None, precision=None):
+        """Experimental semivariograms (ss3d_variogram).  field: numpy array (nreal, nz, ny, nx) of
+        float64 or float32 in host memory, or a device address with device_shape=(nreal, nz, ny, nx)
+        and precision given.  dirs: [(sx, sy, sz, nlag), ...].  Returns gam[r, d, l-1]."""
+        dirs = [tuple(int(v) for v in d) for d in dirs]
+        darr = np.array(dirs, dtype=np.int32).reshape(-1, 4)
+        nlagmax = max(1, int(darr[:, 3].max()))
+        if device_shape is None:
+            field = np.ascontiguousarray(field)
+            if field.ndim == 3:
+                field = field[None]
+            prec = FP64 if field.dtype == np.float64 else FP32
+            if field.dtype not in (np.float64, np.float32):
+                raise TypeError("field must be float64 or float32")
+            nreal, nz, ny, nx = field.shape
+            ptr, on_dev = field.ctypes.data, 0
+        else:
+            nreal, nz, ny, nx = device_shape
+            prec, ptr, on_dev = precision, field, 1
+        gam = np.zeros((nreal, len(dirs), nlagmax))
+        check(lib().ss3d_variogram(self.handle, _vp(ptr), on_dev, prec, nx, ny, nz, nreal, len(dirs),
+                                   darr.ctypes.data_as(_vp), gam.ctypes.data_as(_vp)))
+        return gam
 
     def autotune(self, spec, nx, ny, nz, igauss, reps=3):
         """Pick the fastest radix plan of each FP32 pass for this grid; returns {kind: rm}."""
@@ -376,3 +405,14 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def npairs(nx, ny, nz, lx, ly, lz):
+    """Node pairs at lag (lx, ly, lz) (ss3d_npairs)."""
+    return lib().ss3d_npairs(nx, ny, nz, lx, ly, lz)
+
+
+def covariance(sts, nugget, hx, hy, hz):
+    """Model covariance C(h) (ss3d_covariance); the model variogram is C(0) - C(h)."""
+    arr, n = structures(sts)
+    return lib().ss3d_covariance(arr, n, nugget, hx, hy, hz)

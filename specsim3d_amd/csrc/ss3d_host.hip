@@ -235,6 +235,7 @@ struct ss3d_context {
     // autotuned radix plan of the FP32 passes: (profiling kind, log2 length) -> largest radix log
     std::map<std::pair<int, int>, int> tune;
     std::vector<hipEvent_t> event_pool;
+    DevBuf vario_in, vario_out, vario_np, vario_part;  // ss3d_variogram work buffers
     ~ss3d_context() {
         for (auto& r : prof) {
             (void)hipEventDestroy(r.e0);
@@ -1163,3 +1164,4 @@ int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
 
 #include "ss3d_server.inc"
 #include "ss3d_batch.inc"
+#include "ss3d_vario.inc"

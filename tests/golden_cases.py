@@ -33,3 +33,22 @@ def stdout_stats(c):
             parts = ln.replace("=", " ").split()
             out.append((float(parts[3]), float(parts[5])))
     return out
+
+
+VARIO = os.path.join(GOLDEN, "vario")
+
+
+def vario_cases():
+    with open(os.path.join(VARIO, "vario.json")) as fh:
+        return json.load(fh)["cases"]
+
+
+def vario_field(name):
+    return np.load(os.path.join(VARIO, name + ".npy"), allow_pickle=False)
+
+
+def vario_table(name):
+    """The reference's <case>_vario.csv: (header, rows as lists of strings)."""
+    with open(os.path.join(VARIO, name + "_vario.csv")) as fh:
+        lines = fh.read().splitlines()
+    return lines[0].split(","), [ln.split(",") for ln in lines[1:]]

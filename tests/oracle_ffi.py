@@ -66,6 +66,10 @@ def lib():
                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                   ctypes.POINTER(SpecDiag), ctypes.POINTER(ctypes.c_int)]
+        L.or_npair_count.restype = ctypes.c_int64
+        L.or_npair_count.argtypes = [ctypes.c_int] * 6
+        L.or_vario_lag.restype = ctypes.c_double
+        L.or_vario_lag.argtypes = [_dp] + [ctypes.c_int] * 6
         _lib = L
     return _lib
 
@@ -176,3 +180,14 @@ def simulate(sts, nugget, dx, dy, dz, nx, ny, nz, ipad, iseed, nreal, igauss):
     return dict(res=res.reshape(nreal, nz, ny, nx), mean=mean, var=var, maximag=mi,
                 diag=dict(max_imag=d.max_imag, max_real=d.max_real, c0=d.c0, spos=d.spos, sneg=d.sneg),
                 np=tuple(npo))
+
+
+def npair_count(nx, ny, nz, lx, ly, lz):
+    return lib().or_npair_count(nx, ny, nz, lx, ly, lz)
+
+
+def vario_lag(f, lx, ly, lz):
+    """f: one field, shape (nz, ny, nx) (x fastest in memory)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, ny, nx = f.shape
+    return lib().or_vario_lag(_p(f), nx, ny, nz, lx, ly, lz)

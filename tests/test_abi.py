@@ -70,3 +70,19 @@ def test_argument_errors_are_reported():
         S.periodic_grid([(4, 1, 1, 1, 1, 0, 0, 0)], 1, 1, 1, 8, 8, 8, 1)
     with pytest.raises(S.SS3DError):
         S.periodic_grid([(1, 1, 1, 1, 1, 0, 0, 0)], 1, 1, 1, 8, 8, 8, 2)
+
+
+def test_host_covariance_and_pairs_equal_oracle():
+    """ss3d_covariance / ss3d_npairs are host functions: no GPU needed."""
+    import oracle_ffi as O
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        nst = int(rng.integers(1, 4))
+        sts = [(int(rng.integers(1, 4)), float(rng.uniform(0.1, 1)), *map(float, rng.uniform(1, 50, 3)),
+                *map(float, rng.uniform(-180, 360, 3))) for _ in range(nst)]
+        h = [float(v) for v in rng.uniform(-40, 40, 3)] if rng.random() < 0.9 else [0.0, 0.0, 0.0]
+        nug = float(rng.uniform(0, 0.5))
+        assert S.covariance(sts, nug, *h) == O.covariance(sts, nug, *h)
+        dims = [int(v) for v in rng.integers(1, 20, 3)]
+        lag = [int(v) for v in rng.integers(-20, 20, 3)]
+        assert S.npairs(*dims, *lag) == O.npair_count(*dims, *lag)

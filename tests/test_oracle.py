@@ -206,3 +206,69 @@ def test_fixed_amplitudes_give_deterministic_variance():
     assert np.std(r["var"]) < 1e-12
     r = O.simulate([(2, 1.0, 5, 5, 5, 0, 0, 0)], 0.0, 1, 1, 1, 16, 16, 16, 0, 4711, 6, 1)
     assert np.std(r["var"]) > 1e-3
+
+
+def _numpy_vario(f, lx, ly, lz):
+    """Brute force over shifted slices (f: nz, ny, nx): 0.5 mean (f(x+l) - f(x))^2."""
+    nz, ny, nx = f.shape
+    if abs(lx) >= nx or abs(ly) >= ny or abs(lz) >= nz:
+        return 0.0
+    a = f[max(0, -lz):nz - max(0, lz), max(0, -ly):ny - max(0, ly), max(0, -lx):nx - max(0, lx)]
+    b = f[max(0, lz):nz - max(0, -lz), max(0, ly):ny - max(0, -ly), max(0, lx):nx - max(0, -lx)]
+    return 0.5 * float(np.mean((b - a) ** 2))
+
+
+@pytest.mark.parametrize("lag", [(1, 0, 0), (0, 3, 0), (0, 0, 2), (2, -1, 1), (-3, 2, -1), (9, 0, 0), (0, 0, 5)])
+def test_vario_lag_brute_force(lag):
+    """vario_lag (f90:405-431) equals the shifted-slice mean, and is even in the lag."""
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal((5, 6, 9))
+    g = O.vario_lag(f, *lag)
+    assert g == pytest.approx(_numpy_vario(f, *lag), rel=1e-13, abs=0.0)
+    assert O.vario_lag(f, *(-v for v in lag)) == pytest.approx(g, rel=1e-13)
+    nz, ny, nx = f.shape
+    exp_np = (nx - abs(lag[0])) * (ny - abs(lag[1])) * (nz - abs(lag[2])) if all(
+        abs(v) < n for v, n in zip(lag, (nx, ny, nz))) else 0
+    assert O.npair_count(nx, ny, nz, *lag) == exp_np
+
+
+def test_vario_known_answers():
+    """A linear ramp along x has gamma(l) = l^2/2 exactly; a constant field has 0; no pairs gives 0."""
+    f = np.tile(np.arange(7.0), (3, 4, 1))
+    for l in range(1, 7):
+        assert O.vario_lag(f, l, 0, 0) == 0.5 * l * l
+    assert O.vario_lag(np.full((3, 4, 7), 2.5), 1, 1, 1) == 0.0
+    assert O.vario_lag(f, 7, 0, 0) == 0.0 and O.npair_count(7, 4, 3, 0, 4, 0) == 0
+
+
+@pytest.mark.parametrize("name", sorted(G.vario_cases()))
+def test_vario_matches_reference_table(name):
+    """The oracle's variograms of the reference's FP64 realizations reproduce the reference's
+    --vario table (es14.6): pairs, lag distance, model, per-realization values, mean and sd."""
+    c = G.vario_cases()[name]
+    f = G.vario_field(name)
+    head, rows = G.vario_table(name)
+    assert head[:10] == ["dir", "sx", "sy", "sz", "lag", "h", "npairs", "gamma_model", "gamma_mean", "gamma_sd"]
+    nx, ny, nz = c["grid"]
+    dx, dy, dz = c["spacing"]
+    c0 = c["nugget"] + sum(s[1] for s in c["structures"])
+    expected = []
+    for d, (sx, sy, sz, nlag) in enumerate(c["dirs"], 1):
+        for il in range(1, nlag + 1):
+            lx, ly, lz = il * sx, il * sy, il * sz
+            npairs = O.npair_count(nx, ny, nz, lx, ly, lz)
+            if npairs == 0:
+                continue
+            g = np.array([O.vario_lag(f[r], lx, ly, lz) for r in range(c["nreal"])])
+            h = math.sqrt((lx * dx) ** 2 + (ly * dy) ** 2 + (lz * dz) ** 2)
+            model = c0 - O.covariance(c["structures"], c["nugget"], lx * dx, ly * dy, lz * dz)
+            sd = float(np.sqrt(np.sum((g - g.mean()) ** 2) / (len(g) - 1))) if len(g) > 1 else 0.0
+            expected.append((d, sx, sy, sz, il, h, npairs, model, g.mean(), sd, g))
+    assert len(rows) == len(expected)
+    for row, (d, sx, sy, sz, il, h, npairs, model, gm, sd, g) in zip(rows, expected):
+        assert [int(v) for v in row[:5]] == [d, sx, sy, sz, il] and int(row[6]) == npairs
+        got = [float(v) for v in row[5:6] + row[7:]]
+        want = [h, model, gm, sd] + (list(g) if c["iper"] else [])
+        assert len(got) == len(want)
+        for a, b in zip(got, want):
+            assert a == pytest.approx(b, rel=6e-7, abs=1e-300), (name, row)
