@@ -16,7 +16,7 @@
 !
 ! Options beyond the reference: --fp32 (FP32 realizations), --no-grid-cap (periodic grids
 ! above the reference's 2^27-node limit).  Not part of this engine: conditioning (--cond,
-! icond=1) and variograms (--vario); both stop with a message.
+! icond=1), which stops with a message.  --vario computes the experimental variograms on the GPU.
 !=======================================================================
 module ss3d_writers
   use iso_c_binding, only: c_double
@@ -154,7 +154,7 @@ program specsim3d
   implicit none
   integer, parameter :: dp = c_double, maxst = 10
   ! run description
-  character(len=256) :: parfile, outbase, fname, line, arg
+  character(len=256) :: parfile, outbase, fname, line, arg, varfile
   real(dp) :: org(3), spc(3), nugget
   integer :: nx, ny, nz, nst, iseed, nreal, ipad, igauss, icode, ival, nout, icond
   type(ss3d_structure_t) :: st(maxst)
@@ -170,7 +170,7 @@ program specsim3d
   real(4), allocatable, target :: buf32(:, :)
   real(dp), allocatable :: zmean(:), zvar(:), zimag(:)
   integer :: r0, cnt, chunk
-  integer(8) :: t0, t_spec, t_real, t_end, rate
+  integer(8) :: t0, t_spec, t_real, t_vario, t_end, rate
 
   call system_clock(t0, rate)
   parfile = 'SPECSIM3D.PAR'
@@ -192,6 +192,7 @@ program specsim3d
     case ('--vario')
       ia = ia + 1
       if (ia > na) stop '--vario needs a variogram parameter file'
+      call get_command_argument(ia, varfile)
       f_vario = .true.
     case ('--cond')
       ia = ia + 1
@@ -206,9 +207,6 @@ program specsim3d
     end select
     ia = ia + 1
   end do
-  if (f_vario) then
-    write(*,*) 'variograms (--vario) are not part of this engine: compute them from the .bin output'; stop 1
-  end if
 
   inquire(file=trim(parfile), exist=exists)
   if (.not. exists) then
@@ -351,9 +349,14 @@ program specsim3d
       '  max|Im|=', zimag(i)
   end do
   i = ss3d_spectrum_destroy(spec)
-  i = ss3d_context_destroy(ctx)
   call system_clock(t_real)
   write(*, '(a,f8.2,a)') ' time realizations         : ', real(t_real - t_spec)/real(rate), ' s'
+  if (f_vario) then
+    call vario_report(trim(varfile))
+    call system_clock(t_vario)
+    write(*, '(a,f8.2,a)') ' time variograms           : ', real(t_vario - t_real)/real(rate), ' s'
+  end if
+  i = ss3d_context_destroy(ctx)
 
   ! ---- outputs (same files and order as the reference)
   if (o_bin) call write_bin(trim(outbase)//'.bin', res, int(ncell), nreal)
@@ -373,6 +376,84 @@ program specsim3d
   write(*, '(a,f8.2,a)') ' elapsed time: ', real(t_end - t0)/real(rate), ' s'
 
 contains
+
+  ! Experimental variograms of every realization along the directions of the variogram file,
+  ! their ensemble mean and sd and the model variogram: the report and the <base>_vario.csv table
+  ! of vario_report (f90:434-529), with the pair sums computed on the GPU (ss3d_variogram).
+  ! File: ndir iperreal / ndir lines sx sy sz nlag (lag k = k*(sx,sy,sz) cells).
+  subroutine vario_report(vfile)
+    character(len=*), intent(in) :: vfile
+    integer :: ndir, iper, id, il, r, nlagmax, nfit, u
+    integer(c_int), allocatable :: dirs(:, :)
+    real(dp), allocatable, target :: gam(:, :, :)
+    real(dp) :: c0, gm, gsd, gth, h, rms, mx, e, hx, hy, hz
+    integer(8) :: np
+    character(len=512) :: vname
+    open(newunit=u, file=vfile, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(*,*) 'cannot open variogram file: ', vfile; return
+    end if
+    read(u, *) ndir, iper
+    allocate(dirs(4, ndir))
+    do id = 1, ndir
+      read(u, *) dirs(1, id), dirs(2, id), dirs(3, id), dirs(4, id)
+    end do
+    close(u)
+    nlagmax = max(1, maxval(dirs(4, :)))
+    allocate(gam(nlagmax, ndir, max(nreal, 1)))
+    gam = 0.0_dp
+    if (nreal > 0) call ss3d_check(ss3d_variogram(ctx, c_loc(res), 0_c_int, SS3D_FP64, int(nx, c_int), &
+                                   int(ny, c_int), int(nz, c_int), int(nreal, c_int), int(ndir, c_int), &
+                                   dirs, gam), 'variograms')
+    c0 = nugget
+    do i = 1, nst
+      c0 = c0 + st(i)%sill
+    end do
+    vname = trim(outbase)//'_vario.csv'
+    open(newunit=u, file=trim(vname), status='replace', action='write')
+    write(u, '(a)', advance='no') 'dir,sx,sy,sz,lag,h,npairs,gamma_model,gamma_mean,gamma_sd'
+    if (iper == 1) then
+      do r = 1, nreal
+        write(u, '(a,i0)', advance='no') ',gamma_r', r
+      end do
+    end if
+    write(u, '(a)') ''
+    write(*, '(a)') ' variograms (Fortran): direction  lags  RMS(mean-model)  max|err|   [rising part, ' // &
+                    'gamma_model < 0.95 sill]'
+    do id = 1, ndir
+      rms = 0.0_dp; mx = 0.0_dp; nfit = 0
+      do il = 1, dirs(4, id)
+        np = ss3d_npairs(int(nx, c_int), int(ny, c_int), int(nz, c_int), int(il*dirs(1, id), c_int), &
+                         int(il*dirs(2, id), c_int), int(il*dirs(3, id), c_int))
+        if (np == 0) cycle
+        hx = il*dirs(1, id)*spc(1); hy = il*dirs(2, id)*spc(2); hz = il*dirs(3, id)*spc(3)
+        h = sqrt(hx*hx + hy*hy + hz*hz)
+        gth = c0 - ss3d_covariance(st, int(nst, c_int), nugget, hx, hy, hz)
+        gm = sum(gam(il, id, 1:nreal))/real(nreal, dp)
+        if (nreal > 1) then
+          gsd = sqrt(sum((gam(il, id, 1:nreal) - gm)**2)/real(nreal - 1, dp))
+        else
+          gsd = 0.0_dp
+        end if
+        write(u, '(i0,3(",",i0),",",i0,",",es14.6,",",i0,3(",",es14.6))', advance='no') &
+              id, dirs(1, id), dirs(2, id), dirs(3, id), il, h, np, gth, gm, gsd
+        if (iper == 1) then
+          do r = 1, nreal
+            write(u, '(",",es14.6)', advance='no') gam(il, id, r)
+          end do
+        end if
+        write(u, '(a)') ''
+        if (gth < 0.95_dp*c0) then
+          e = gm - gth
+          rms = rms + e*e; mx = max(mx, abs(e)); nfit = nfit + 1
+        end if
+      end do
+      if (nfit > 0) rms = sqrt(rms/real(nfit, dp))
+      write(*, '(a,i3,3i4,i6,2f12.5)') '   dir', id, dirs(1, id), dirs(2, id), dirs(3, id), dirs(4, id), rms, mx
+    end do
+    close(u)
+    write(*, '(a,a)') ' variogram table: ', trim(vname)
+  end subroutine vario_report
 
   function upper(s) result(u)
     character(len=*), intent(in) :: s

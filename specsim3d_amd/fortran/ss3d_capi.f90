@@ -11,7 +11,8 @@ module ss3d_capi
   private
   public :: ss3d_structure_t, ss3d_spec_diag_t, SS3D_FP64, SS3D_FP32
   public :: ss3d_context_create, ss3d_context_destroy, ss3d_periodic_grid, ss3d_spectrum_create, &
-            ss3d_spectrum_destroy, ss3d_realize, ss3d_draws_per_realization, ss3d_check
+            ss3d_spectrum_destroy, ss3d_realize, ss3d_draws_per_realization, ss3d_check, &
+            ss3d_variogram, ss3d_npairs, ss3d_covariance
 
   integer(c_int), parameter :: SS3D_FP64 = 0, SS3D_FP32 = 1
 
@@ -76,6 +77,28 @@ module ss3d_capi
       integer(c_int), value :: iseed, igauss, nreal, nx, ny, nz, precision
       integer(c_int64_t), value :: r0
       real(c_double), intent(out) :: mean(*), var(*), maximag(*)
+    end function
+
+    ! variograms (vario_lag f90:405-431): gam(nlagmax, ndir, nreal) in Fortran order
+    integer(c_int) function ss3d_variogram(ctx, field, on_device, precision, nx, ny, nz, nreal, ndir, dirs, gam) &
+        bind(C, name='ss3d_variogram')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, field
+      integer(c_int), value :: on_device, precision, nx, ny, nz, nreal, ndir
+      integer(c_int), intent(in) :: dirs(*)
+      real(c_double), intent(out) :: gam(*)
+    end function
+
+    integer(c_int64_t) function ss3d_npairs(nx, ny, nz, lx, ly, lz) bind(C, name='ss3d_npairs')
+      import :: c_int, c_int64_t
+      integer(c_int), value :: nx, ny, nz, lx, ly, lz
+    end function
+
+    real(c_double) function ss3d_covariance(st, nst, nugget, hx, hy, hz) bind(C, name='ss3d_covariance')
+      import :: c_int, c_double, ss3d_structure_t
+      type(ss3d_structure_t), intent(in) :: st(*)
+      integer(c_int), value :: nst
+      real(c_double), value :: nugget, hx, hy, hz
     end function
 
     integer(c_int) function ss3d_last_error_c(buf, length) bind(C, name='ss3d_last_error')

@@ -146,3 +146,36 @@ def test_fp32_option(tmp_path):
     got = np.fromfile(tmp_path / "g.bin", dtype="<f4").astype(np.float64)
     want = G.field("exp_nug_gauss").ravel()
     assert np.sqrt(np.mean((got - want) ** 2) / np.mean(want ** 2)) < 1e-5
+
+
+@pytest.mark.parametrize("name", sorted(G.vario_cases()))
+def test_vario_table_and_report(tmp_path, name):
+    """--vario: the <base>_vario.csv table and the stdout report of vario_report (f90:434-529)
+    against the reference's own, for FP64 realizations of the same run."""
+    c = G.vario_cases()[name]
+    say(f"[driver] vario {name}")
+    (tmp_path / "v.par").write_text(par_text(c["origin"], c["spacing"], c["grid"], c["nugget"], c["structures"],
+                                             c["iseed"], c["nreal"], c["ipad"], c["igauss"], base=name, codes=[3]))
+    shutil.copy(os.path.join(G.VARIO, name + "_v.par"), tmp_path / "vv.par")
+    r = run(EXE, str(tmp_path), "v.par", "--vario", "vv.par")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert any(ln.startswith(" time variograms           :") for ln in r.stdout.splitlines())
+    head, rows = G.vario_table(name)
+    lines = (tmp_path / (name + "_vario.csv")).read_text().splitlines()
+    assert lines[0].split(",") == head
+    got = [ln.split(",") for ln in lines[1:]]
+    assert len(got) == len(rows)
+    for g, w in zip(got, rows):
+        assert g[:5] == w[:5] and g[6] == w[6]  # dir, step, lag, pairs
+        assert g[5] == w[5] and g[7] == w[7]    # h and the model variogram: same host arithmetic
+        for a, b in zip(g[8:], w[8:]):          # ensemble and per-realization values
+            assert float(a) == pytest.approx(float(b), rel=2e-6, abs=1e-12), (g, w)
+    want = [ln for ln in c["stdout"] if "vario" in ln or ln.startswith("   dir")]
+    have = [ln for ln in strip_times(r.stdout) if "vario" in ln or ln.startswith("   dir")]
+    assert len(have) == len(want)
+    for h, w in zip(have, want):
+        if h.startswith("   dir"):
+            assert h.split()[:6] == w.split()[:6]
+            assert [float(v) for v in h.split()[6:]] == pytest.approx([float(v) for v in w.split()[6:]], abs=2e-5)
+        else:
+            assert h == w
