@@ -55,6 +55,11 @@ class Hip:
     def free(self, p):
         self.L.hipFree(ctypes.c_void_p(p))
 
+    def copy_d2h(self, dst, src, n):
+        rc = self.L.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(n), 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy d2h failed: {rc}")
+
     def copy_d2d(self, dst, src, n):
         rc = self.L.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(n), 3)
         if rc != 0:
@@ -193,6 +198,65 @@ def uneven_batch(S, eng, seed, njobs=32, nlanes=4, reps=5):
                                              for r in reqs})}
 
 
+def large_job(S, eng, hip, nreal=32, checks=(0, 31)):
+    """One large job: 512x512x256 output in a 1024x1024x512 periodic grid, gaussian model
+    exp(-3r^2), sill 1, ranges 150/60/20, azimuth 120, dip -20, ipad=1, igauss=1, nreal FP32
+    realizations kept in HBM.  Reports the realization throughput, the FP32-vs-FP64 relative RMS
+    error of the checked realizations (FP64 drop-in path on the same draws) and the device
+    variograms of every realization along the 3 grid axes."""
+    import numpy as np
+
+    sts = [(3, 1.0, 150.0, 60.0, 20.0, 120.0, -20.0, 0.0)]
+    nx, ny, nz = 512, 512, 256
+    grid = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    ncell = nx * ny * nz
+    t0 = time.perf_counter()
+    spec = eng.spectrum(sts, 0.0, 1.0, 1.0, 1.0, *grid)
+    eng.synchronize()
+    spec_s = time.perf_counter() - t0
+    plan = eng.autotune(spec, nx, ny, nz, 1, reps=1)
+    out = hip.malloc(nreal * ncell * 4)
+    stats = hip.malloc(nreal * 3 * 8)
+    try:
+        eng.realize_device(spec, 4711, 1, 0, nreal, nx, ny, nz, S.FP32, out, stats)  # warm-up (same work)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.realize_device(spec, 4711, 1, 0, nreal, nx, ny, nz, S.FP32, out, stats)
+        eng.synchronize()
+        t_real = time.perf_counter() - t0
+        # FP32 vs FP64 on the same draws
+        errs = []
+        for r in checks:
+            f64, _, _, _ = eng.realize(spec, 4711, 1, r, 1, nx, ny, nz, S.FP64)
+            f32 = np.empty(ncell, dtype=np.float32)
+            hip.copy_d2h(f32.ctypes.data, out + r * ncell * 4, ncell * 4)
+            d = f32.astype(np.float64) - f64.ravel()
+            errs.append(float(np.sqrt(np.mean(d * d) / np.mean(f64 * f64))))
+        # variograms along x, y, z of every realization (device field)
+        dirs = [(1, 0, 0, 128), (0, 1, 0, 64), (0, 0, 1, 32)]
+        t0 = time.perf_counter()
+        gam = eng.variogram(out, dirs, device_shape=(nreal, nz, ny, nx), precision=S.FP32)
+        t_var = time.perf_counter() - t0
+    finally:
+        hip.free(out)
+        hip.free(stats)
+        spec.close()
+    c0 = 1.0
+    mean_g = gam.mean(axis=0)
+    model = [[c0 - S.covariance(sts, 0.0, l * d[0], l * d[1], l * d[2]) for l in (1, d[3] // 2, d[3])] for d in dirs]
+    return {"workload": "512x512x256 (1024x1024x512 periodic), gaussian sill 1, ranges 150/60/20, az 120, dip -20, "
+                        f"ipad=1, igauss=1, {nreal} FP32 realizations in HBM",
+            "periodic_grid": list(grid), "spectrum_s": round(spec_s, 3), "radix_plan": plan,
+            "realizations_s": round(t_real, 4), "gcells_s": round(nreal * ncell / t_real / 1e9, 2),
+            "fp32_vs_fp64_rel_rms": {str(r): e for r, e in zip(checks, errs)},
+            "variogram": {"dirs": dirs, "seconds": round(t_var, 4),
+                          "pairs_per_s": float(sum(S.npairs(nx, ny, nz, l * d[0], l * d[1], l * d[2])
+                                                   for d in dirs for l in range(1, d[3] + 1)) * nreal / t_var),
+                          "mean_at_lags_1_half_max": [[round(float(mean_g[k, l - 1]), 4) for l in (1, d[3] // 2, d[3])]
+                                                      for k, d in enumerate(dirs)],
+                          "model_at_same_lags": [[round(v, 4) for v in m] for m in model]}}
+
+
 def load_traffic(kind):
     path = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if not os.path.exists(path):
@@ -214,6 +278,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
@@ -283,6 +348,7 @@ def main():
     spec.close()
 
     uneven = None if args.no_batch else uneven_batch(S, eng, 3)
+    large = None if args.no_large else large_job(S, eng, hip)
     serving = None
     if not args.no_serve:
         sustained = serve(S, eng, args.serve_rate, 6.0, 1)
@@ -323,6 +389,7 @@ def main():
         "spectrum_ms": round(spec_ms, 2),
         "autotune": {"largest_radix_log2": plan, "seconds": round(tune_s, 2)},
         "uneven_batch": uneven,
+        "large_job": large,
         "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
                         "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
