@@ -258,12 +258,14 @@ def large_job(S, eng, hip, nreal=32, checks=(0, 31)):
 
 
 def load_traffic(kind):
+    """HBM bytes per launch of a pass kind measured with rocprofv3 counters (profiles/traffic_c2.json:
+    FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB device copy)."""
     path = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         t = json.load(fh)
-    v = t.get(kind)
+    v = t.get("kinds", {}).get(kind)
     return None if v is None else v.get("hbm_bytes_per_launch")
 
 
