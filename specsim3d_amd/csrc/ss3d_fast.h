@@ -548,14 +548,28 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
             const f2* __restrict__ src = reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride +
                                          (int64_t)blockIdx.x * P * a.in_row;
             const uint32_t base = (uint32_t)q * (uint32_t)a.in_row;
-            // column 0 = g0 + i gN (packed Nyquist column): X[0] = Re, X[N] = Im
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) {
-                const int k = g + r * G;
-                xa[r] = src[base + k];
-                xb[r] = k == 0 ? mk(xa[r].y, 0.f) : src[base + (N - k)];
-                if (k == 0) xa[r].y = 0.f;
+            for (int r = 0; r < EPT; ++r) xa[r] = src[base + g + r * G];
+            if constexpr (G <= 64) {
+                // a row lies inside one wave: X[N-k] for k = g + r G is register EPT-1-r of lane G-g of the
+                // row (g >= 1), or this lane's register EPT-r (g = 0, r >= 1) -- read once, exchanged
+                const int lane = threadIdx.x & 63;
+                const int plane = lane - g + ((G - g) & (G - 1));
+#pragma unroll
+                for (int r = 0; r < EPT; ++r) {
+                    const f2 t = xa[EPT - 1 - r];
+                    const f2 sh = mk(__shfl(t.x, plane, 64), __shfl(t.y, plane, 64));
+                    xb[r] = g != 0 ? sh : (r == 0 ? mk(xa[0].y, 0.f) : xa[(EPT - r) & (EPT - 1)]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < EPT; ++r) {
+                    const int k = g + r * G;
+                    xb[r] = k == 0 ? mk(xa[r].y, 0.f) : src[base + (N - k)];
+                }
             }
+            // column 0 = g0 + i gN (packed Nyquist column): X[0] = Re, X[N] = Im
+            if (g == 0) xa[0].y = 0.f;
         } else {  // IN_FILLH: the 1D grid (npy = npz = 1) filled in place
             const uint32_t rb = __ldg(a.rbase + b);
 #pragma unroll
