@@ -90,7 +90,6 @@ __device__ __forceinline__ uint32_t site_state(const PassArgs& a, uint32_t rb, c
 
 // FP64 coefficient exactly as f90:1309-1332 (own amplitude at every point).
 __device__ __forceinline__ cpx<double> coef64(double amp, uint32_t st, int igauss, const DrawSite& d) {
-    const double twopi = 2.0 * 3.14159265358979323846;  // 2.0d0*pi
     const double s05 = 0.70710678118654752440;           // sqrt(0.5d0)
     cpx<double> c;
     if (!igauss) {
@@ -99,19 +98,21 @@ __device__ __forceinline__ cpx<double> coef64(double amp, uint32_t st, int igaus
             c.x = (st <= 1073741823u) ? amp : -amp;
             c.y = 0.0;
         } else {
-            double u = (double)st / 2147483647.0;
-            double phi = twopi * u;
+            double u = (double)st * (1.0 / 2147483647.0);
             double sn, cs;
-            sincos(phi, &sn, &cs);
+            sincospi(2.0 * u, &sn, &cs);
             c.x = amp * cs;
             c.y = d.conj ? amp * (-sn) : amp * sn;
         }
     } else {
-        double u1 = (double)st / 2147483647.0;
-        double u2 = (double)pm_mul(st, kPmA) / 2147483647.0;
+        // u = state/2147483647 (f90:199) as a product with the rounded reciprocal, and
+        // cos/sin(2 pi u2) through sincospi (exact argument reduction): both within an ulp or two
+        // of the reference's division and cos(2.0d0*pi*u2), far inside the FP64 path's tolerance
+        double u1 = (double)st * (1.0 / 2147483647.0);
+        double u2 = (double)pm_mul(st, kPmA) * (1.0 / 2147483647.0);
         double rr = sqrt(-2.0 * log(u1));
         double sn, cs;
-        sincos(twopi * u2, &sn, &cs);
+        sincospi(2.0 * u2, &sn, &cs);
         double g1 = rr * cs, g2 = rr * sn;
         if (d.selfc) {
             c.x = amp * g1;
