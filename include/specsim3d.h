@@ -131,6 +131,9 @@ int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
  * SS3D_OPT_PIPELINE (default 0): FP32 realizations run the fill pass of batch k+1 on a second
  * stream while the remaining passes of batch k run, double-buffering the fill output. */
 #define SS3D_OPT_PIPELINE 1
+/* SS3D_OPT_FP64_GENERIC (default 0): FP64 realizations through the generic (runtime-length) pass
+ * kernel instead of the compile-time kernels; both compute the same values bit for bit. */
+#define SS3D_OPT_FP64_GENERIC 2
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
 /* ---------------------------------------------------------------- request loop

@@ -20,6 +20,7 @@ EXE_PATH = os.path.join(PKG, "bin", "specsim3d")
 FP64 = 0
 FP32 = 1
 OPT_PIPELINE = 1
+OPT_FP64_GENERIC = 2
 
 ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device"}
 

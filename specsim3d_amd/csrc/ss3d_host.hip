@@ -222,6 +222,7 @@ struct ss3d_context {
     DevBuf tw32n;
     Lane main;               // the context's own lane (main.stream is `stream`)
     bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
+    bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
     // per-pass event timing (ss3d_context_profile)
@@ -385,7 +386,8 @@ struct Pass {
     int nblocks;
     PassArgs a;
     int kind;      // profiling kind (see ss3d_context_profile)
-    int fast = 0;  // FP32 fast kernels: 1 strided, 2 C2R x pass, 3 one-point x axis
+    int fast = 0;  // FP32 fast kernels: 1 strided, 2 C2R x pass, 3 one-point x axis;
+                   // FP64 compile-time kernels: 4 x pass (fill + rows), 5 strided
     int rm = 5;    // FP32 fast kernels: largest radix 2^rm
 };
 
@@ -437,6 +439,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     if (p.fast == 1) P = fast_strided_tile(p.a.logn);
     if (p.fast == 2) P = fast_c2r_rows(p.a.logn, p.rm);
     if (p.fast == 3) P = 256;
+    if (p.fast == 4 || p.fast == 5) P = fast64_tile(p.fast, p.a.logn);
     if (p.a.axis == 0) {
         p.nblocks = (int)((p.a.nrows + P - 1) / P);
     } else {
@@ -475,6 +478,8 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
         e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, strm);
     else if (p.fast == 3)
         e = launch_fast_x1(p.nblocks, nbatch, p.a, strm);
+    else if (p.fast == 4 || p.fast == 5)
+        e = launch_fast64(p.fast, p.a.logn, p.out, p.nblocks, nbatch, p.a, strm);
     else
         e = launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, strm);
     if (e != hipSuccess) return fail(SS3D_EHIP, "pass launch (prec %d in %d out %d nt %d): %s", p.prec, p.in, p.out,
@@ -815,6 +820,10 @@ int enqueue_batch(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int ig
         p1.a.nrows = npy * npz;
         p1.a.n_out = nx;
         p1.a.tw = tw_ptr(c->tw64i, lx, 16);
+        // compile-time kernels (same arithmetic) where the lengths allow
+        const bool f64ok = !c->fp64_generic;
+        auto fits = [](int l) { return l >= 1 && l <= kMaxFast64Log; };
+        if (f64ok && (has_y || has_z) && fits(lx)) p1.fast = 4;
         if (has_y || has_z) {
             p1.a.out = L.wa.p;
             p1.a.out_row = nx;
@@ -842,6 +851,7 @@ int enqueue_batch(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int ig
             p2.a.out_sT = (int64_t)nx * ny;
             p2.a.out_bstride = has_z ? b2 : ncrop;
             p2.a.tw = tw_ptr(c->tw64i, ly, 16);
+            if (f64ok && fits(ly)) p2.fast = 5;
             SS3D_TRY(run_pass(c, p2, nb, L.stream, L.partial));
             last = &p2;
         }
@@ -861,6 +871,7 @@ int enqueue_batch(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int ig
             p3.a.out_sT = nx;
             p3.a.out_bstride = ncrop;
             p3.a.tw = tw_ptr(c->tw64i, lz, 16);
+            if (f64ok && fits(lz)) p3.fast = 5;
             SS3D_TRY(run_pass(c, p3, nb, L.stream, L.partial));
             last = &p3;
         }
@@ -1142,6 +1153,10 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         HIP_TRY(hipStreamSynchronize(c->main.stream2));
         c->pipeline = value != 0;
+        return SS3D_OK;
+    }
+    if (option == SS3D_OPT_FP64_GENERIC) {
+        c->fp64_generic = value != 0;
         return SS3D_OK;
     }
     return fail(SS3D_EINVAL, "unknown option %d", option);

@@ -102,6 +102,13 @@ hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int nblocks
 hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 
+// FP64 realization passes with compile-time lengths 2^1 .. 2^kMaxFast64Log (ss3d_fast64.h), bit
+// for bit the arithmetic of launch_pass64.  kind 4: x pass (fill + transform of rows, OUT_STORE),
+// kind 5: strided y/z pass (IN_LOAD, OUT_STORE or OUT_REAL).  fast64_tile: rows or columns per block.
+constexpr int kMaxFast64Log = 12;
+int fast64_tile(int kind, int L);
+hipError_t launch_fast64(int kind, int L, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
+
 // LDS bytes a pass with nt threads uses (dynamic shared memory).
 inline size_t pass_lds_bytes(int nt, size_t elem_bytes) {
     size_t e = (size_t)nt * kEPT;

@@ -2,9 +2,56 @@
 // realization path) and the reduction finalizers.  Built with -ffp-contract=off: every
 // product and sum is rounded separately, as in the reference build, so butterflies,
 // covariance and coefficients follow the reference's arithmetic.
-#include "ss3d_pass.h"
+#include "ss3d_fast64.h"
 
 namespace ss3d {
+
+// ---------------------------------------------------------------- compile-time FP64 passes
+namespace {
+template <typename K>
+hipError_t launch_f64(K k, size_t lds, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s, bool& attr) {
+    if (lds > 65536 && !attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, dim3(nblocks, nbatch), dim3(nt), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int L>
+hipError_t f64_L(int kind, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    constexpr int G = (1 << L) >> f64::el(L);
+    if (kind == 4) {
+        static bool attr = false;
+        return launch_f64(f64::rows_fill_kernel<L>, f64::rows_lds(L), G * f64::rows_P(L), nblocks, nbatch, a, s, attr);
+    }
+    if (out == OUT_STORE) {
+        static bool attr = false;
+        return launch_f64(f64::strided_kernel<L, OUT_STORE>, f64::strided_lds(L), G * f64::strided_P(L), nblocks,
+                          nbatch, a, s, attr);
+    }
+    static bool attr = false;
+    return launch_f64(f64::strided_kernel<L, OUT_REAL>, f64::strided_lds(L), G * f64::strided_P(L), nblocks, nbatch,
+                      a, s, attr);
+}
+
+template <int... Ls>
+hipError_t f64_dispatch(int L, int kind, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
+                        std::integer_sequence<int, Ls...>) {
+    hipError_t e = hipErrorInvalidValue;
+    ((L == Ls + 1 ? (e = f64_L<Ls + 1>(kind, out, nblocks, nbatch, a, s), 0) : 0), ...);
+    return e;
+}
+}  // namespace
+
+int fast64_tile(int kind, int L) { return kind == 4 ? f64::rows_P(L) : f64::strided_P(L); }
+
+hipError_t launch_fast64(int kind, int L, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    if (L < 1 || L > kMaxFast64Log) return hipErrorInvalidValue;
+    if (kind == 5 && out != OUT_STORE && out != OUT_REAL) return hipErrorInvalidValue;
+    return f64_dispatch(L, kind, out, nblocks, nbatch, a, s, std::make_integer_sequence<int, kMaxFast64Log>{});
+}
 
 hipError_t launch_pass64(int in, int out, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using R = double;

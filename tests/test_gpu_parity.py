@@ -198,3 +198,31 @@ def test_stream_pipeline_gives_identical_results(eng):
         eng.set_option(S.OPT_PIPELINE, 0)
     sp.close()
     assert np.array_equal(a, b) and np.array_equal(ma, mb) and np.array_equal(va, vb)
+
+
+FP64_GRIDS = [(64, 64, 16, 1, 1), (40, 30, 1, 1, 0), (13, 7, 5, 0, 1), (1, 16, 8, 0, 1), (33, 1, 1, 1, 1),
+              (256, 256, 256, 1, 1), (20, 12, 300, 1, 0), (6, 1400, 3, 1, 1)]
+
+
+@pytest.mark.parametrize("grid", FP64_GRIDS)
+def test_fp64_compile_time_kernels_equal_generic(eng, grid):
+    """The compile-time FP64 passes compute the generic pass kernel's values bit for bit."""
+    nx, ny, nz, ipad, igauss = grid
+    sts = [(2, 0.8, 9.0, 5.0, 3.0, 30.0, 10.0, 0.0), (1, 0.2, 25.0, 25.0, 12.0, 0.0, 0.0, 0.0)]
+    npg = S.periodic_grid(sts, 1, 1, 1, nx, ny, nz, ipad)
+    say(f"[fp64 kernels] grid {grid} periodic {npg}")
+    sp = eng.spectrum(sts, 0.1, 1, 1, 1, *npg)
+    nreal = 2 if nx * ny * nz > 1 << 22 else 3
+    try:
+        f1, m1, v1, i1 = eng.realize(sp, 9191, igauss, 5, nreal, nx, ny, nz, S.FP64)
+        eng.set_option(S.OPT_FP64_GENERIC, 1)
+        try:
+            f2, m2, v2, i2 = eng.realize(sp, 9191, igauss, 5, nreal, nx, ny, nz, S.FP64)
+        finally:
+            eng.set_option(S.OPT_FP64_GENERIC, 0)
+    finally:
+        sp.close()
+    assert np.array_equal(f1, f2), np.abs(f1 - f2).max()
+    # statistics: the same sums over differently shaped blocks
+    assert np.allclose(m1, m2, rtol=0, atol=1e-14) and np.allclose(v1, v2, rtol=1e-13)
+    assert np.allclose(i1, i2, rtol=0, atol=1e-15)
