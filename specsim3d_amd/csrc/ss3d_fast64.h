@@ -242,6 +242,118 @@ __global__ __launch_bounds__((1 << L) / (1 << el(L)) * rows_P(L)) void rows_fill
     }
 }
 
+// ------------------------------------------------------------ x pass over Hermitian row pairs
+// Row R = j + npy k and its partner row Rp = (-j, -k): for R < Rp every point of R draws and
+// the point (-i) of Rp takes the conjugate of that draw (f90:1305-1334).  A unit of G threads
+// (a line slot each) computes the draws of R once, applies the amplitudes of both rows, and
+// transforms R, then Rp, in its LDS row.  The partner's value for slot g is the draw of slot
+// G-g (register EPT-1-r), moved across the unit's lanes (the unit lies inside one wave).
+// Self rows (R = Rp: j in {0, npy/2}, k in {0, npz/2}) take the pointwise path in blocks of
+// their own.  Units: lower rows enumerated in closed form, self rows after them.
+template <int L>
+__global__ __launch_bounds__((1 << L) / (1 << el(L)) * rows_P(L)) void rows_pair_kernel(PassArgs a) {
+    constexpr int N = 1 << L, EPT = 1 << el(L), G = N / EPT, P = rows_P(L), NT = G * P, M = nch(L);
+    static_assert(G <= 64, "a unit lies inside one wave");
+    constexpr int ROW = N + (N >> 3);
+    c64* lds = reinterpret_cast<c64*>(ss3d_smem);
+    c64* twl = lds + (size_t)ROW * P;
+    const int tid = threadIdx.x;
+    const int g = tid % G, q = tid / G;
+    const int npy = a.npy, npz = a.npz;
+    const int nlow = npy >= 2 ? npy / 2 - 1 : 0, nsp = npz >= 2 ? 2 : 1;
+    const int npairs = nsp * nlow + (npz >= 2 ? (npz / 2 - 1) * npy : 0);
+    const int nbp = (npairs + P - 1) / P;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const bool pairs = blk < nbp;  // block-uniform
+    const int u = (pairs ? blk : blk - nbp) * P + q;
+    const int nself = (npy >= 2 ? 2 : 1) * nsp;
+    const bool active = pairs ? u < npairs : u < nself;
+    const int b = blockIdx.y;
+    const c64* __restrict__ tw = reinterpret_cast<const c64*>(a.tw);
+    if constexpr (M > 0) {
+        for (int m = tid; m < N / 2; m += NT) twl[m] = tw[m];
+        __syncthreads();
+    }
+    int j = 0, k = 0;
+    if (pairs) {
+        if (u < nsp * nlow) {
+            k = u / nlow == 0 ? 0 : npz / 2;
+            j = 1 + u % nlow;
+        } else {
+            const int u2 = u - nsp * nlow;
+            k = 1 + u2 / npy;
+            j = u2 % npy;
+        }
+    } else {
+        const int ny2 = npy >= 2 ? 2 : 1;
+        j = (u % ny2) * (npy / 2);
+        k = (u / ny2) * (npz / 2);
+    }
+    const int row = j + npy * k;
+    const int rowp = ((npy - j) & (npy - 1)) + npy * ((npz - k) & (npz - 1));
+    const uint32_t rb = __ldg(a.rbase + b);
+    c64 v[EPT], w[EPT];
+    if (active) {
+        const double* __restrict__ amp = a.amp64 + (int64_t)N * row;
+        if (pairs) {
+            const double* __restrict__ ampp = a.amp64 + (int64_t)N * rowp;
+            const uint32_t tr = pm_mul(rb, __ldg(a.rowf + row));
+            Draw64 z[EPT];
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+                const int pos = g + r * G;
+                z[r] = draw64(pm_mul(tr, __ldg(a.txp + pos)), a.igauss);
+                v[r] = coef_of_draw(__ldg(amp + pos), z[r], a.igauss, false);
+            }
+            const int lane = threadIdx.x & 63;
+            const int plane = lane - g + ((G - g) & (G - 1));
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+                const Draw64 t = z[EPT - 1 - r];
+                const Draw64 sh{__shfl(t.a, plane, 64), __shfl(t.b, plane, 64)};
+                const Draw64 zp = g != 0 ? sh : z[(EPT - r) & (EPT - 1)];
+                w[r] = coef_of_draw(__ldg(ampp + g + r * G), zp, a.igauss, true);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+                const int pos = g + r * G;
+                const DrawSite d = classify(pos, j, k, N, npy, npz);
+                v[r] = coef64(__ldg(amp + pos), site_state(a, rb, d), a.igauss, d);
+                w[r] = c64{0.0, 0.0};
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) v[r] = w[r] = c64{0.0, 0.0};
+    }
+    c64* lrow = lds + (size_t)q * ROW;
+    constexpr int RCl = M ? ch_rc(L, M - 1) : 0, Sl = M ? ch_s(L, M - 1) : 0, NBl = EPT >> RCl;
+    const int nout = a.n_out;
+    auto finish = [&](c64* x, int rw) {
+        transform<L, G>(x, g, lrow, [](int pos) { return pos + (pos >> 3); }, TwL{twl});
+        if (active) {
+            c64* __restrict__ dst = reinterpret_cast<c64*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)rw * a.out_row;
+#pragma unroll
+            for (int i = 0; i < NBl; ++i)
+#pragma unroll
+                for (int bb = 0; bb < (1 << RCl); ++bb) {
+                    const int pos = g + G * i + (bb << Sl);
+                    if (pos < nout) {
+                        using d2 = double __attribute__((ext_vector_type(2)));
+                        const c64 zz = x[i * (1 << RCl) + bb];
+                        __builtin_nontemporal_store((d2){zz.x, zz.y}, reinterpret_cast<d2*>(dst + pos));
+                    }
+                }
+        }
+    };
+    finish(v, row);
+    if (pairs) {  // block-uniform
+        __syncthreads();  // LDS reads of row R done
+        finish(w, rowp);
+    }
+}
+
 // dynamic LDS of a tile: exchange buffer + twiddle table
 constexpr size_t strided_lds(int L) { return ((size_t)(1 << L) * strided_P(L) + (1 << L) / 2) * sizeof(c64); }
 constexpr size_t rows_lds(int L) { return ((size_t)((1 << L) + ((1 << L) >> 3)) * rows_P(L) + (1 << L) / 2) * sizeof(c64); }

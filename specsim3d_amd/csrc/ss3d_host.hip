@@ -440,7 +440,9 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     if (p.fast == 2) P = fast_c2r_rows(p.a.logn, p.rm);
     if (p.fast == 3) P = 256;
     if (p.fast == 4 || p.fast == 5) P = fast64_tile(p.fast, p.a.logn);
-    if (p.a.axis == 0) {
+    if (p.fast == 4) {
+        p.nblocks = fast64_rows_blocks(p.a.logn, p.a.npy, p.a.npz);
+    } else if (p.a.axis == 0) {
         p.nblocks = (int)((p.a.nrows + P - 1) / P);
     } else {
         p.a.ntiles0 = (p.a.n0 + P - 1) / P;

@@ -107,6 +107,7 @@ hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_
 // kind 5: strided y/z pass (IN_LOAD, OUT_STORE or OUT_REAL).  fast64_tile: rows or columns per block.
 constexpr int kMaxFast64Log = 12;
 int fast64_tile(int kind, int L);
+int fast64_rows_blocks(int L, int npy, int npz);  // blocks of a kind-4 launch
 hipError_t launch_fast64(int kind, int L, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 
 // LDS bytes a pass with nt threads uses (dynamic shared memory).

@@ -23,8 +23,15 @@ template <int L>
 hipError_t f64_L(int kind, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     constexpr int G = (1 << L) >> f64::el(L);
     if (kind == 4) {
-        static bool attr = false;
-        return launch_f64(f64::rows_fill_kernel<L>, f64::rows_lds(L), G * f64::rows_P(L), nblocks, nbatch, a, s, attr);
+        if constexpr (G <= 64) {  // row pairs: one draw per Hermitian pair
+            static bool attr = false;
+            return launch_f64(f64::rows_pair_kernel<L>, f64::rows_lds(L), G * f64::rows_P(L), nblocks, nbatch, a, s,
+                              attr);
+        } else {
+            static bool attr = false;
+            return launch_f64(f64::rows_fill_kernel<L>, f64::rows_lds(L), G * f64::rows_P(L), nblocks, nbatch, a, s,
+                              attr);
+        }
     }
     if (out == OUT_STORE) {
         static bool attr = false;
@@ -46,6 +53,17 @@ hipError_t f64_dispatch(int L, int kind, int out, int nblocks, int nbatch, const
 }  // namespace
 
 int fast64_tile(int kind, int L) { return kind == 4 ? f64::rows_P(L) : f64::strided_P(L); }
+
+// blocks of the x pass: row-pair kernel (lengths whose rows fit one wave) or one row per unit
+int fast64_rows_blocks(int L, int npy, int npz) {
+    const int P = f64::rows_P(L);
+    const int G = (1 << L) >> f64::el(L);
+    if (G > 64) return (npy * npz + P - 1) / P;
+    const int nlow = npy >= 2 ? npy / 2 - 1 : 0, nsp = npz >= 2 ? 2 : 1;
+    const int npairs = nsp * nlow + (npz >= 2 ? (npz / 2 - 1) * npy : 0);
+    const int nself = (npy >= 2 ? 2 : 1) * nsp;
+    return (npairs + P - 1) / P + (nself + P - 1) / P;
+}
 
 hipError_t launch_fast64(int kind, int L, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     if (L < 1 || L > kMaxFast64Log) return hipErrorInvalidValue;

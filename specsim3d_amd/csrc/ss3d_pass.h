@@ -88,41 +88,57 @@ __device__ __forceinline__ uint32_t site_state(const PassArgs& a, uint32_t rb, c
     return pm_mul(pm_mul(rb, row), __ldg(a.txp + d.i));
 }
 
-// FP64 coefficient exactly as f90:1309-1332 (own amplitude at every point).
-__device__ __forceinline__ cpx<double> coef64(double amp, uint32_t st, int igauss, const DrawSite& d) {
-    const double s05 = 0.70710678118654752440;           // sqrt(0.5d0)
+// The draw of a site, before its amplitude: igauss=0 (cos, sin) of the phase 2 pi u;
+// igauss=1 the Box-Muller pair (g1, g2) (f90:1309-1332, gauss_pair f90:200-205).
+struct Draw64 {
+    double a, b;
+};
+__device__ __forceinline__ Draw64 draw64(uint32_t st, int igauss) {
+    // u = state/2147483647 (f90:199) as a product with the rounded reciprocal, and
+    // cos/sin(2 pi u) through sincospi (exact argument reduction): both within an ulp or two
+    // of the reference's division and cos(2.0d0*pi*u), far inside the FP64 path's tolerance
+    double sn, cs;
+    if (!igauss) {
+        const double u = (double)st * (1.0 / 2147483647.0);
+        sincospi(2.0 * u, &sn, &cs);
+        return {cs, sn};
+    }
+    const double u1 = (double)st * (1.0 / 2147483647.0);
+    const double u2 = (double)pm_mul(st, kPmA) * (1.0 / 2147483647.0);
+    const double rr = sqrt(-2.0 * log(u1));
+    sincospi(2.0 * u2, &sn, &cs);
+    return {rr * cs, rr * sn};
+}
+
+// Coefficient of a point that is not self-conjugate from its site's draw: amp (cos, +-sin) or
+// amp (g1, +-g2) sqrt(0.5), the sign negative at the partner of the drawing point (conj).
+__device__ __forceinline__ cpx<double> coef_of_draw(double amp, Draw64 z, int igauss, bool conj) {
+    const double s05 = 0.70710678118654752440;  // sqrt(0.5d0)
     cpx<double> c;
     if (!igauss) {
-        if (d.selfc) {
-            // urand() < 0.5d0  <=>  state <= 1073741823 (exact integer form)
-            c.x = (st <= 1073741823u) ? amp : -amp;
-            c.y = 0.0;
-        } else {
-            double u = (double)st * (1.0 / 2147483647.0);
-            double sn, cs;
-            sincospi(2.0 * u, &sn, &cs);
-            c.x = amp * cs;
-            c.y = d.conj ? amp * (-sn) : amp * sn;
-        }
+        c.x = amp * z.a;
+        c.y = conj ? amp * (-z.b) : amp * z.b;
     } else {
-        // u = state/2147483647 (f90:199) as a product with the rounded reciprocal, and
-        // cos/sin(2 pi u2) through sincospi (exact argument reduction): both within an ulp or two
-        // of the reference's division and cos(2.0d0*pi*u2), far inside the FP64 path's tolerance
-        double u1 = (double)st * (1.0 / 2147483647.0);
-        double u2 = (double)pm_mul(st, kPmA) * (1.0 / 2147483647.0);
-        double rr = sqrt(-2.0 * log(u1));
-        double sn, cs;
-        sincospi(2.0 * u2, &sn, &cs);
-        double g1 = rr * cs, g2 = rr * sn;
-        if (d.selfc) {
-            c.x = amp * g1;
-            c.y = 0.0;
-        } else {
-            c.x = amp * g1 * s05;
-            c.y = d.conj ? amp * (-g2) * s05 : amp * g2 * s05;
-        }
+        c.x = amp * z.a * s05;
+        c.y = conj ? amp * (-z.b) * s05 : amp * z.b * s05;
     }
     return c;
+}
+
+// FP64 coefficient exactly as f90:1309-1332 (own amplitude at every point).
+__device__ __forceinline__ cpx<double> coef64(double amp, uint32_t st, int igauss, const DrawSite& d) {
+    if (d.selfc) {
+        cpx<double> c;
+        if (!igauss) {
+            // urand() < 0.5d0  <=>  state <= 1073741823 (exact integer form)
+            c.x = (st <= 1073741823u) ? amp : -amp;
+        } else {
+            c.x = amp * draw64(st, 1).a;
+        }
+        c.y = 0.0;
+        return c;
+    }
+    return coef_of_draw(amp, draw64(st, igauss), igauss, d.conj);
 }
 
 // Covariance at a lag, f90:153-178, in the reference's operation order.
