@@ -201,7 +201,8 @@ def test_stream_pipeline_gives_identical_results(eng):
 
 
 FP64_GRIDS = [(64, 64, 16, 1, 1), (40, 30, 1, 1, 0), (13, 7, 5, 0, 1), (1, 16, 8, 0, 1), (33, 1, 1, 1, 1),
-              (256, 256, 256, 1, 1), (20, 12, 300, 1, 0), (6, 1400, 3, 1, 1)]
+              (256, 256, 256, 1, 1), (20, 12, 300, 1, 0), (6, 1400, 3, 1, 1), (700, 3, 2, 1, 1),
+              (2000, 5, 1, 0, 0), (3, 1, 40, 1, 1), (2, 2, 2, 0, 1)]
 
 
 @pytest.mark.parametrize("grid", FP64_GRIDS)
