@@ -257,6 +257,84 @@ def large_job(S, eng, hip, nreal=32, checks=(0, 31)):
                           "model_at_same_lags": [[round(v, 4) for v in m] for m in model]}}
 
 
+def writer_leg(S, eng, spec, nreal):
+    """Output files.  (1) The workload's nreal realizations to a raw float32 .bin file, FP32 and
+    FP64 fields: streamed by the library's writer (device compute, device-to-host copy and file
+    writes overlapped) vs the serial order of the reference (every realization to host memory,
+    then the file).  (2) GSLIB text (.out, es15.7, every realization of a node per line) of 8
+    FP64 realizations of 128x128x64 through the writer, and the reference program's own run of the
+    same case (its elapsed time minus its spectrum and realization timers is its writer)."""
+    import numpy as np
+
+    nx, ny, nz = GRID
+    ncell = nx * ny * nz
+    wd = tempfile.mkdtemp(prefix="ss3d_out_")
+    out = {"workload": f"{nx}x{ny}x{nz}, {nreal} realizations to a raw float32 .bin (files in {wd}, no fsync)"}
+    try:
+        for prec, name in ((S.FP32, "fp32"), (S.FP64, "fp64")):
+            path = os.path.join(wd, name + ".bin")
+            t0 = time.perf_counter()
+            with S.Writer(eng, {"bin": path}, nx, ny, nz, nreal, prec) as w:
+                done = 0
+                while done < nreal:
+                    c = min(w.slot, nreal - done)
+                    w.realize(spec, ISEED, IGAUSS, c)
+                    done += c
+            t_stream = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            head = np.fromfile(path, dtype="<f4", count=ncell)
+            os.remove(path)
+            t0 = time.perf_counter()
+            f = eng.realize(spec, ISEED, IGAUSS, 0, nreal, nx, ny, nz, prec)[0]
+            t_host = time.perf_counter() - t0
+            (f if prec == S.FP32 else f.astype(np.float32)).tofile(path)
+            t_serial = time.perf_counter() - t0
+            same = bool(np.array_equal(head, f[0].astype(np.float32).ravel()))
+            del f
+            os.remove(path)
+            out[name] = {"streamed_s": round(t_stream, 3), "streamed_GB_s": round(size / t_stream / 1e9, 2),
+                         "serial_s": round(t_serial, 3), "serial_realize_to_host_s": round(t_host, 3),
+                         "file_GB": round(size / 1e9, 3), "speedup": round(t_serial / t_stream, 2),
+                         "first_realization_equal": same}
+        # GSLIB text
+        g = (128, 128, 64)
+        nr = 8
+        grid = S.periodic_grid(STRUCTS, 1.0, 1.0, 1.0, *g, 1)
+        sp = eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *grid)
+        path = os.path.join(wd, "g.out")
+        t0 = time.perf_counter()
+        with S.Writer(eng, {"gslib": path}, *g, nr, S.FP64) as w:
+            w.realize(sp, ISEED, IGAUSS, nr)
+        t_text = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        os.remove(path)
+        sp.close()
+        text = {"case": "128x128x64, 8 FP64 realizations, es15.7", "seconds": round(t_text, 3),
+                "MB_s": round(size / t_text / 1e6, 1), "file_MB": round(size / 1e6, 1)}
+        exe = os.path.join(ROOT, "oracle", "_ref", "specsim3d")
+        if os.path.exists(exe):
+            s0 = STRUCTS[0]
+            with open(os.path.join(wd, "g.par"), "w") as fh:
+                fh.write("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d %d 1 %d\ng\n1\n1\n0\n"
+                         % (*g, NUGGET, s0[0], *s0[1:], ISEED, nr, IGAUSS))
+            thr = min(os.cpu_count() or 1, 16)
+            r = subprocess.run([exe, "g.par"], cwd=wd, capture_output=True, text=True,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(thr)), timeout=600)
+            if r.returncode == 0:
+                num = lambda pat: float(re.search(pat, r.stdout).group(1))
+                tw = num(r"elapsed time:\s+([\d.]+)") - num(r"realizations\s+:\s+([\d.]+)") - \
+                    num(r"covariance\+spectrum:\s+([\d.]+)")
+                text["reference_writer_s"] = round(tw, 2)
+                text["reference_threads"] = thr
+                text["speedup_vs_reference_writer"] = round(tw / t_text, 1)
+        out["gslib_text"] = text
+    finally:
+        import shutil
+
+        shutil.rmtree(wd, ignore_errors=True)
+    return out
+
+
 def load_traffic(kind):
     """HBM bytes per launch of a pass kind measured with rocprofv3 counters (profiles/traffic_c2.json:
     FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB device copy)."""
@@ -281,6 +359,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-writer", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
@@ -347,6 +426,7 @@ def main():
         prof64 = eng.profile_read()
         hip.free(out64)
     hip.free(stats)
+    writer = None if args.no_writer else writer_leg(S, eng, spec, nreal)
     spec.close()
 
     uneven = None if args.no_batch else uneven_batch(S, eng, 3)
@@ -392,6 +472,7 @@ def main():
         "autotune": {"largest_radix_log2": plan, "seconds": round(tune_s, 2)},
         "uneven_batch": uneven,
         "large_job": large,
+        "output_files": writer,
         "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
                         "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -31,6 +31,7 @@ extern "C" {
 #define SS3D_ENOMEM 3   /* device or host allocation failed */
 #define SS3D_EHIP 4     /* HIP runtime error */
 #define SS3D_ENODEV 5   /* no usable GPU */
+#define SS3D_EIO 6      /* an output file could not be opened or written (message names it) */
 
 /* arithmetic of the per-realization path */
 #define SS3D_FP64 0 /* drop-in: C2C x->y->z in FP64, reference butterfly order (f90:1335) */
@@ -174,6 +175,60 @@ int ss3d_server_info(ss3d_server* s, int64_t* queued, int64_t* running, int64_t*
 double ss3d_server_clock(const ss3d_server* s);
 /* Waits for every submitted request, then releases the lanes and the cache. */
 int ss3d_server_destroy(ss3d_server* s);
+
+/* ---------------------------------------------------------------- output files
+ * The reference's writers (src/specsim3d.f90:307-391 write_vti / write_vtk_bin, and the output
+ * block f90:1357-1414), as a writer that streams files while realizations are computed.
+ * Files (NULL or vtk_count = 0: not written), each byte-identical to the reference's:
+ *   bin_path        raw float32 little-endian, realizations one after the other, x fastest
+ *                   (f90:1362-1376);
+ *   vti_path        VTK XML ImageData with every realization as appended raw float32 blocks,
+ *                   each preceded by its UInt64 byte count (f90:307-359, 1377-1380);
+ *   vtk_base        binary legacy VTK (big-endian float32) of realizations 1..vtk_count, named
+ *                   <vtk_base>_<r>.vtk (f90:362-391, 1370-1373);
+ *   gslib_path      GSLIB grid file: title, nreal, sim1..simN, then one line per node with every
+ *                   realization in es15.7 (f90:1381-1396);
+ *   vtk_ascii_path  ASCII legacy VTK of realization 1, es13.5 (f90:1397-1413).
+ * Values written as float32 are real(x, 4) of the FP64 field (round to nearest even). */
+typedef struct ss3d_outputs {
+    const char* bin_path;
+    const char* vti_path;
+    const char* vtk_base;
+    int32_t vtk_count;
+    int32_t reserved; /* 0 */
+    const char* gslib_path;
+    const char* vtk_ascii_path;
+    double origin[3];  /* xmin, ymin, zmin */
+    double spacing[3]; /* dx, dy, dz */
+} ss3d_outputs;
+
+typedef struct ss3d_writer ss3d_writer;
+/* Creates (replaces) the files and writes every header; all file offsets follow from the grid and
+ * nreal.  ctx may be NULL for a writer fed only by ss3d_writer_put.  precision: of the fields the
+ * writer receives (SS3D_FP64 as the reference's `res`, or SS3D_FP32). */
+int ss3d_writer_open(ss3d_context* ctx, const ss3d_outputs* o, int nx, int ny, int nz, int nreal, int precision,
+                     ss3d_writer** out);
+/* Largest number of realizations per ss3d_writer_realize call (about 256 MiB of output). */
+int ss3d_writer_slot(const ss3d_writer* w);
+/* Computes the next nreal (1..slot) realizations of the stream (iseed, igauss), as ss3d_realize
+ * does, and returns their statistics (mean, var, maximag: nreal doubles each, may be NULL).  Their
+ * output goes to the files in the background: a copy stream moves them to pinned host memory and a
+ * writer thread writes them while the following calls compute.  *dev_field (may be NULL) receives
+ * the device address of these realizations (nx*ny*nz*nreal values of the writer's precision),
+ * valid for work ordered on the context's stream before the next call.  Must be called in order
+ * until nreal realizations have been given. */
+int ss3d_writer_realize(ss3d_writer* w, const ss3d_spectrum* spec, int iseed, int igauss, int nreal, double* mean,
+                        double* var, double* maximag, const void** dev_field);
+/* Writes the next nreal realizations from host memory (the writer's precision); returns when the
+ * buffer may be reused. */
+int ss3d_writer_put(ss3d_writer* w, const void* host_field, int nreal);
+/* Waits for every pending write, writes the GSLIB and ASCII VTK files (several threads, fixed-width
+ * lines at known offsets) and the .vti trailer, closes the files and frees the writer, also on
+ * failure.  Returns the first error of any write. */
+int ss3d_writer_close(ss3d_writer* w);
+/* Fortran ESw.d of n values into out (n*w characters, no terminator), as the writers print them:
+ * correctly rounded digits (ties to even), E+dd exponents, +ddd beyond 99, '*' fill on overflow. */
+int ss3d_format_es(const double* v, int64_t n, int w, int d, char* out);
 
 /* ---------------------------------------------------------------- variograms
  * Experimental semivariograms, replacing vario_lag (src/specsim3d.f90:405-431) as vario_report

@@ -10,6 +10,7 @@ Reference: src/specsim3d.f90 of stebond/specsim3d (the hot block f90:1234-1345).
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -22,7 +23,7 @@ FP32 = 1
 OPT_PIPELINE = 1
 OPT_FP64_GENERIC = 2
 
-ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device"}
+ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device", 6: "I/O error"}
 
 
 class SS3DError(RuntimeError):
@@ -58,6 +59,14 @@ class Completion(ctypes.Structure):
 class SpecDiag(ctypes.Structure):
     _fields_ = [("max_imag", ctypes.c_double), ("max_real", ctypes.c_double), ("c0", ctypes.c_double),
                 ("spos", ctypes.c_double), ("sneg", ctypes.c_double)]
+
+
+class Outputs(ctypes.Structure):
+    """ss3d_outputs: the files of one run (the reference's writers, f90:307-391, 1357-1414)."""
+    _fields_ = [("bin_path", ctypes.c_char_p), ("vti_path", ctypes.c_char_p), ("vtk_base", ctypes.c_char_p),
+                ("vtk_count", ctypes.c_int32), ("reserved", ctypes.c_int32), ("gslib_path", ctypes.c_char_p),
+                ("vtk_ascii_path", ctypes.c_char_p), ("origin", ctypes.c_double * 3),
+                ("spacing", ctypes.c_double * 3)]
 
 
 _lib = None
@@ -111,6 +120,14 @@ SIGNATURES = {
     "ss3d_covariance": (ctypes.c_double, [ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double]),
     "ss3d_context_profile_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
+    "ss3d_writer_open": (ctypes.c_int, [_vp, ctypes.POINTER(Outputs), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ss3d_writer_slot": (ctypes.c_int, [_vp]),
+    "ss3d_writer_realize": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                           ctypes.POINTER(_vp)]),
+    "ss3d_writer_put": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int]),
+    "ss3d_writer_close": (ctypes.c_int, [_vp]),
+    "ss3d_format_es": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
 }
 
 NPASS_KINDS = 9
@@ -206,6 +223,7 @@ class Engine:
 
     def __init__(self, device=0):
         self.handle = _vp()
+        self._deps = weakref.WeakSet()  # spectra, servers, batches, writers: closed before the context
         check(lib().ss3d_context_create(device, ctypes.byref(self.handle)))
 
     def spectrum(self, sts, nugget, dx, dy, dz, npx, npy, npz):
@@ -215,7 +233,9 @@ class Engine:
         check(lib().ss3d_spectrum_create(self.handle, arr, n, nugget, dx, dy, dz, npx, npy, npz, ctypes.byref(h),
                                          ctypes.byref(d)))
         diag = dict(max_imag=d.max_imag, max_real=d.max_real, c0=d.c0, spos=d.spos, sneg=d.sneg)
-        return Spectrum(self, h, (npx, npy, npz), diag)
+        sp = Spectrum(self, h, (npx, npy, npz), diag)
+        self._deps.add(sp)
+        return sp
 
     def realize(self, spec, iseed, igauss, r0, nreal, nx, ny, nz, precision=FP64):
         """Realizations r0..r0+nreal-1 -> (field[nreal, nz, ny, nx], mean, var, maximag)."""
@@ -287,6 +307,12 @@ class Engine:
 
     def close(self):
         if self.handle:
+            deps = list(self._deps)
+            for d in [d for d in deps if not isinstance(d, Spectrum)] + [d for d in deps if isinstance(d, Spectrum)]:
+                try:
+                    d.close()
+                except SS3DError:
+                    pass
             lib().ss3d_context_destroy(self.handle)
             self.handle = None
 
@@ -295,6 +321,68 @@ class Engine:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Writer:
+    """Streaming writer of the reference's output files (ss3d_writer_*).
+
+    paths: dict with any of bin, vti, vtk_base (+ vtk_count), gslib, vtk_ascii; origin and
+    spacing as in the parameter file.  engine=None gives a writer fed only by put()."""
+
+    def __init__(self, engine, paths, nx, ny, nz, nreal, precision=FP64, origin=(0.0, 0.0, 0.0),
+                 spacing=(1.0, 1.0, 1.0)):
+        enc = lambda k: os.fsencode(paths[k]) if paths.get(k) else None
+        self._o = Outputs(enc("bin"), enc("vti"), enc("vtk_base"), int(paths.get("vtk_count", 0)), 0, enc("gslib"),
+                          enc("vtk_ascii"), (ctypes.c_double * 3)(*origin), (ctypes.c_double * 3)(*spacing))
+        self.shape = (nx, ny, nz)
+        self.nreal = nreal
+        self.precision = precision
+        self.handle = _vp()
+        check(lib().ss3d_writer_open(engine.handle if engine else None, ctypes.byref(self._o), nx, ny, nz, nreal,
+                                     precision, ctypes.byref(self.handle)))
+        self.slot = lib().ss3d_writer_slot(self.handle)
+        if engine:
+            engine._deps.add(self)
+
+    def realize(self, spec, iseed, igauss, count):
+        """Next `count` realizations on the device; returns (mean, var, maximag, device address)."""
+        mean, var, mi = np.empty(count), np.empty(count), np.empty(count)
+        dev = _vp()
+        check(lib().ss3d_writer_realize(self.handle, spec.handle, iseed, igauss, count, mean.ctypes.data_as(_dp),
+                                        var.ctypes.data_as(_dp), mi.ctypes.data_as(_dp), ctypes.byref(dev)))
+        return mean, var, mi, dev.value
+
+    def put(self, field):
+        """Next realizations from a host array (nreal, nz, ny, nx) of the writer's precision."""
+        dt = np.float64 if self.precision == FP64 else np.float32
+        field = np.ascontiguousarray(field, dtype=dt)
+        nx, ny, nz = self.shape
+        n = field.size // (nx * ny * nz)
+        check(lib().ss3d_writer_put(self.handle, field.ctypes.data_as(_vp), n))
+
+    def close(self):
+        if self.handle:
+            h, self.handle = self.handle, None
+            check(lib().ss3d_writer_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        if exc[0] is None:
+            self.close()
+        elif self.handle:
+            h, self.handle = self.handle, None
+            lib().ss3d_writer_close(h)
+
+
+def format_es(values, w, d):
+    """Fortran ESw.d of each value, as the library's writers print them -> list of str."""
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    buf = ctypes.create_string_buffer(max(1, v.size * w))
+    check(lib().ss3d_format_es(v.ctypes.data_as(_dp), v.size, w, d, buf))
+    raw = buf.raw[: v.size * w].decode("ascii")
+    return [raw[i * w:(i + 1) * w] for i in range(v.size)]
 
 
 def simulate(engine, sts, nugget, dx, dy, dz, nx, ny, nz, ipad, iseed, nreal, igauss, precision=FP64):
@@ -317,6 +405,7 @@ class Server:
         self.engine = engine
         self.handle = _vp()
         check(lib().ss3d_server_create(engine.handle, nslots, hbm_budget, ctypes.byref(self.handle)))
+        engine._deps.add(self)
         self._keep = {}
 
     @staticmethod
@@ -382,6 +471,7 @@ class Batch:
         arr = (Request * len(self.requests))(*self.requests)
         self.handle = _vp()
         check(lib().ss3d_batch_create(engine.handle, arr, len(self.requests), nlanes, ctypes.byref(self.handle)))
+        engine._deps.add(self)
         self.cells = lib().ss3d_batch_cells(self.handle)
 
     def run(self, graph=True):

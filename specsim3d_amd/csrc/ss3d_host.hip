@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -237,6 +238,9 @@ struct ss3d_context {
     std::map<std::pair<int, int>, int> tune;
     std::vector<hipEvent_t> event_pool;
     DevBuf vario_in, vario_out, vario_np, vario_part;  // ss3d_variogram work buffers
+    // spectra created by ss3d_spectrum_create and not destroyed yet: detached (ctx = null) when the
+    // context is destroyed, so that destroying them afterwards touches no freed context
+    std::set<ss3d_spectrum*> spectra;
     ~ss3d_context() {
         for (auto& r : prof) {
             (void)hipEventDestroy(r.e0);
@@ -252,7 +256,8 @@ struct RngTables {
 };
 
 struct ss3d_spectrum {
-    ss3d_context* ctx = nullptr;
+    ss3d_context* ctx = nullptr;  // null once the context is destroyed
+    int device = 0;
     int npx = 0, npy = 0, npz = 0, hx = 0, hxa = 0;
     DevBuf amp64, amp32h, diagd;
     RngTables rng[2];
@@ -543,10 +548,12 @@ int ss3d_context_destroy(ss3d_context* ctx) {
     if (!ctx) return SS3D_OK;
     (void)hipSetDevice(ctx->device);
     ctx->main.sync();
+    for (ss3d_spectrum* sp : ctx->spectra) sp->ctx = nullptr;
     hipStream_t s = ctx->main.stream, s2 = ctx->main.stream2;
     delete ctx;
     if (s) (void)hipStreamDestroy(s);
     if (s2) (void)hipStreamDestroy(s2);
+    (void)hipGetLastError();  // results of the clean-up calls above are not reported
     return SS3D_OK;
 }
 
@@ -610,6 +617,7 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     HIP_TRY(hipSetDevice(c->device));
     sp.reset(new ss3d_spectrum);
     sp->ctx = c;
+    sp->device = c->device;
     sp->npx = npx;
     sp->npy = npy;
     sp->npz = npz;
@@ -710,6 +718,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     SS3D_TRY(spectrum_build(c, c->main, st, nst, nugget, dx, dy, dz, npx, npy, npz, sp));
     SS3D_TRY(spectrum_read_diag(sp.get(), c->main.stream));
     if (diag) *diag = sp->diag;
+    c->spectra.insert(sp.get());
     *out = sp.release();
     return SS3D_OK;
 }
@@ -749,9 +758,13 @@ int ss3d_context_profile_read(ss3d_context* c, double* ms, double* bytes, int64_
 
 int ss3d_spectrum_destroy(ss3d_spectrum* sp) {
     if (!sp) return SS3D_OK;
-    (void)hipSetDevice(sp->ctx->device);
-    (void)hipStreamSynchronize(sp->ctx->stream);
+    (void)hipSetDevice(sp->device);
+    if (sp->ctx) {  // else its context is gone, and its streams were drained when it was destroyed
+        (void)hipStreamSynchronize(sp->ctx->stream);
+        sp->ctx->spectra.erase(sp);
+    }
     delete sp;
+    (void)hipGetLastError();  // results of the clean-up calls above are not reported
     return SS3D_OK;
 }
 
@@ -763,6 +776,7 @@ int64_t ss3d_spectrum_bytes(const ss3d_spectrum* sp) {
 
 int ss3d_spectrum_amplitude(const ss3d_spectrum* sp, double* host_amp) {
     if (!sp || !host_amp) return fail(SS3D_EINVAL, "null argument");
+    if (!sp->ctx) return fail(SS3D_EINVAL, "the spectrum's context was destroyed");
     HIP_TRY(hipSetDevice(sp->ctx->device));
     HIP_TRY(hipMemcpyAsync(host_amp, sp->amp64.p, (size_t)sp->npx * sp->npy * sp->npz * sizeof(double),
                            hipMemcpyDeviceToHost, sp->ctx->stream));
@@ -785,7 +799,7 @@ int tuned_rm(const ss3d_context* c, int kind, int L) {
 // Enqueue realizations r0..r0+nb-1 into dout (device) and their statistics into dstats.
 int enqueue_batch(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nb, int nx, int ny,
                   int nz, int prec, void* dout, double* dstats) {
-    const int npx = sp->npx, npy = sp->npy, npz = sp->npz, hx = sp->hx;
+    const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
     SS3D_TRY(build_rng(sp, igauss));
     SS3D_TRY(L.rbase.ensure((size_t)nb * 4, L.stream));
@@ -1027,7 +1041,7 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
 int check_realize(ss3d_context* c, const ss3d_spectrum* sp, int igauss, int64_t r0, int nreal, int nx, int ny, int nz,
                   int prec) {
     if (!c || !sp) return fail(SS3D_EINVAL, "null context or spectrum");
-    if (sp->ctx != c) return fail(SS3D_EINVAL, "spectrum belongs to another context");
+    if (sp->ctx != c) return fail(SS3D_EINVAL, "spectrum belongs to another (or a destroyed) context");
     if (igauss != 0 && igauss != 1) return fail(SS3D_EINVAL, "igauss must be 0 or 1");
     if (prec != SS3D_FP64 && prec != SS3D_FP32) return fail(SS3D_EINVAL, "precision must be SS3D_FP64 or SS3D_FP32");
     if (r0 < 0 || nreal < 0) return fail(SS3D_EINVAL, "realization range must be non-negative");
@@ -1182,3 +1196,4 @@ int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
 #include "ss3d_server.inc"
 #include "ss3d_batch.inc"
 #include "ss3d_vario.inc"
+#include "ss3d_output.inc"

@@ -2,7 +2,7 @@
 ! specsim3d -- drop-in driver of the MI355X engine for SPECSIM3D's unconditional path.
 !
 ! Keeps the reference program's process contract (src/specsim3d.f90 of stebond/specsim3d):
-!   command line   specsim3d [file.par] [--gauss] [--bin] [--vtk]            (f90:1111-1141)
+!   command line   specsim3d [file.par] [--gauss] [--bin] [--vtk] [--vario f] (f90:1111-1141)
 !   parameter file optional header up to 'START OF PARAMETERS', grid, model,
 !                  seed line with optional igauss, base name, optional output
 !                  codes and icond                                           (f90:1146-1232)
@@ -11,146 +11,18 @@
 !   files          GSLIB .out, raw float32 .bin, one .vti with every
 !                  realization, binary/ASCII legacy .vtk                     (f90:307-391,1357-1414)
 ! The numerical work runs in libspecsim3d.so (HIP, gfx950) through the ISO_C_BINDING
-! interfaces of module ss3d_capi: grid sizing, the FP64 spectral density, and the
-! realizations (FP64 by default; --fp32 selects the FP32 half-spectrum path).
+! interfaces of module ss3d_capi: grid sizing, the FP64 spectral density, the realizations
+! (FP64 by default; --fp32 selects the FP32 half-spectrum path) and the output files, which the
+! library's writer streams to disk while later realizations are computed (ss3d_writer_*).
 !
 ! Options beyond the reference: --fp32 (FP32 realizations), --no-grid-cap (periodic grids
 ! above the reference's 2^27-node limit).  Not part of this engine: conditioning (--cond,
-! icond=1), which stops with a message.  --vario computes the experimental variograms on the GPU.
+! icond=1), which stops with a message.  --vario computes the experimental variograms on the GPU,
+! chunk by chunk on the device copies of the realizations.
 !=======================================================================
-module ss3d_writers
-  use iso_c_binding, only: c_double
-  implicit none
-  private
-  public :: write_bin, write_vtk_binary, write_vti_file, write_gslib, write_vtk_ascii
-  integer, parameter :: dp = c_double
-  character(len=1), parameter :: lf = achar(10)
-
-contains
-
-  ! raw float32, realizations one after the other, x fastest (f90:1362-1376)
-  subroutine write_bin(fname, res, n, nreal)
-    character(len=*), intent(in) :: fname
-    integer, intent(in) :: n, nreal
-    real(dp), intent(in) :: res(n, nreal)
-    integer :: u, r
-    open(newunit=u, file=trim(fname), status='replace', action='write', access='stream', form='unformatted')
-    do r = 1, nreal
-      write(u) real(res(:, r), 4)
-    end do
-    close(u)
-  end subroutine write_bin
-
-  ! legacy VTK STRUCTURED_POINTS, BINARY big-endian float32 (f90:362-391)
-  subroutine write_vtk_binary(fname, f, nx, ny, nz, org, spc)
-    character(len=*), intent(in) :: fname
-    integer, intent(in) :: nx, ny, nz
-    real(dp), intent(in) :: f(nx*ny*nz), org(3), spc(3)
-    character(len=256) :: ln
-    integer :: u
-    open(newunit=u, file=trim(fname), status='replace', access='stream', form='unformatted', convert='big_endian')
-    write(u) '# vtk DataFile Version 3.0'//lf//'SPECSIM3D realization'//lf//'BINARY'//lf// &
-             'DATASET STRUCTURED_POINTS'//lf
-    write(ln, '(a,3(1x,i0))') 'DIMENSIONS', nx, ny, nz
-    write(u) trim(ln)//lf
-    write(ln, '(a,3(1x,es16.8))') 'ORIGIN', org
-    write(u) trim(ln)//lf
-    write(ln, '(a,3(1x,es16.8))') 'SPACING', spc
-    write(u) trim(ln)//lf
-    write(ln, '(a,i0)') 'POINT_DATA ', int(nx, 8)*int(ny, 8)*int(nz, 8)
-    write(u) trim(ln)//lf//'SCALARS sim float 1'//lf//'LOOKUP_TABLE default'//lf
-    write(u) real(f, 4)
-    write(u) lf
-    close(u)
-  end subroutine write_vtk_binary
-
-  ! one VTK XML ImageData file with every realization, raw appended little-endian float32 with
-  ! UInt64 block sizes (f90:307-359)
-  subroutine write_vti_file(fname, res, nx, ny, nz, nreal, org, spc)
-    character(len=*), intent(in) :: fname
-    integer, intent(in) :: nx, ny, nz, nreal
-    real(dp), intent(in) :: res(nx*ny*nz, nreal), org(3), spc(3)
-    character(len=512) :: ln
-    character(len=48) :: ext
-    integer(8) :: nbytes
-    integer :: u, r
-    integer(1) :: probe(4)
-    probe = transfer(1_4, probe)
-    if (probe(1) /= 1) stop 'write_vti: big-endian machines are not supported'
-    nbytes = 4_8*int(nx, 8)*int(ny, 8)*int(nz, 8)
-    write(ext, '(a,i0,a,i0,a,i0)') '0 ', nx - 1, ' 0 ', ny - 1, ' 0 ', nz - 1
-    open(newunit=u, file=trim(fname), status='replace', access='stream', form='unformatted')
-    write(u) '<?xml version="1.0"?>'//lf
-    write(u) '<VTKFile type="ImageData" version="1.0" byte_order="LittleEndian" header_type="UInt64">'//lf
-    write(ln, '(a,a,a,3(1x,es16.8),a,3(1x,es16.8),a)') '  <ImageData WholeExtent="', trim(ext), '" Origin="', org, &
-      '" Spacing="', spc, '">'
-    write(u) trim(ln)//lf
-    write(ln, '(a,a,a)') '    <Piece Extent="', trim(ext), '">'
-    write(u) trim(ln)//lf
-    write(u) '      <PointData Scalars="sim_1">'//lf
-    do r = 1, nreal
-      write(ln, '(a,i0,a,i0,a)') '        <DataArray type="Float32" Name="sim_', r, '" format="appended" offset="', &
-        (nbytes + 8_8)*int(r - 1, 8), '"/>'
-      write(u) trim(ln)//lf
-    end do
-    write(u) '      </PointData>'//lf//'      <CellData/>'//lf//'    </Piece>'//lf//'  </ImageData>'//lf
-    write(u) '  <AppendedData encoding="raw">'//lf//'   _'
-    do r = 1, nreal
-      write(u) nbytes
-      write(u) real(res(:, r), 4)
-    end do
-    write(u) lf//'  </AppendedData>'//lf//'</VTKFile>'//lf
-    close(u)
-  end subroutine write_vti_file
-
-  ! GSLIB grid file: title, nreal, one name per column, one row per node x fastest (f90:1384-1396)
-  subroutine write_gslib(fname, res, n, nreal)
-    character(len=*), intent(in) :: fname
-    integer, intent(in) :: n, nreal
-    real(dp), intent(in) :: res(n, nreal)
-    integer :: u, r, p
-    open(newunit=u, file=trim(fname), status='replace', action='write')
-    write(u, '(a)') 'SPECSIM3D realizations'
-    write(u, '(i4)') nreal
-    do r = 1, nreal
-      write(u, '(a,i0)') 'sim', r
-    end do
-    do p = 1, n
-      write(u, '(*(es15.7))') res(p, :)
-    end do
-    close(u)
-  end subroutine write_gslib
-
-  ! legacy default: ASCII VTK of realization 1 (f90:1397-1413)
-  subroutine write_vtk_ascii(fname, f, nx, ny, nz, org, spc)
-    character(len=*), intent(in) :: fname
-    integer, intent(in) :: nx, ny, nz
-    real(dp), intent(in) :: f(nx, ny*nz), org(3), spc(3)
-    integer :: u, row
-    open(newunit=u, file=trim(fname), status='replace', action='write')
-    write(u, '(a)') '# vtk DataFile Version 3.0'
-    write(u, '(a)') 'SPECSIM3D realization 1'
-    write(u, '(a)') 'ASCII'
-    write(u, '(a)') 'DATASET STRUCTURED_POINTS'
-    write(u, '(a,3i6)') 'DIMENSIONS', nx, ny, nz
-    write(u, '(a,3es15.7)') 'ORIGIN', org
-    write(u, '(a,3es15.7)') 'SPACING', spc
-    write(u, '(a,i0)') 'POINT_DATA ', nx*ny*nz
-    write(u, '(a)') 'SCALARS sim float 1'
-    write(u, '(a)') 'LOOKUP_TABLE default'
-    do row = 1, ny*nz
-      write(u, '(*(es13.5))') f(:, row)
-    end do
-    close(u)
-  end subroutine write_vtk_ascii
-
-end module ss3d_writers
-
-
 program specsim3d
   use iso_c_binding
   use ss3d_capi
-  use ss3d_writers
   implicit none
   integer, parameter :: dp = c_double, maxst = 10
   ! run description
@@ -166,11 +38,18 @@ program specsim3d
   type(ss3d_spec_diag_t) :: diag
   integer(c_int) :: npx, npy, npz, prec
   integer(8) :: ncell
-  real(dp), allocatable, target :: res(:, :)
-  real(4), allocatable, target :: buf32(:, :)
   real(dp), allocatable :: zmean(:), zvar(:), zimag(:)
   integer :: r0, cnt, chunk
-  integer(8) :: t0, t_spec, t_real, t_vario, t_end, rate
+  integer(8) :: t0, t_spec, t_real, t_vario, t_end, rate, tv0, tv1, tvario
+  ! output files (written by the library's streaming writer)
+  type(c_ptr) :: wr, dfield
+  type(ss3d_outputs_t), target :: outs
+  character(kind=c_char), target :: p_bin(512), p_vti(512), p_vtk(512), p_out(512), p_vtka(512)
+  ! variograms: directions read before the realizations, pair sums taken per chunk on the device
+  logical :: vario_ok
+  integer :: ndir, iper
+  integer(c_int), allocatable :: dirs(:, :)
+  real(dp), allocatable :: gam(:, :, :)
 
   call system_clock(t0, rate)
   parfile = 'SPECSIM3D.PAR'
@@ -321,78 +200,72 @@ program specsim3d
   call system_clock(t_spec)
   write(*, '(a,f8.2,a)') ' time covariance+spectrum: ', real(t_spec - t0)/real(rate), ' s'
 
-  ! ---- realizations
+  ! ---- output files: every header now, realizations streamed as they are computed
   ncell = int(nx, 8)*int(ny, 8)*int(nz, 8)
-  allocate(res(ncell, max(nreal, 1)), zmean(max(nreal, 1)), zvar(max(nreal, 1)), zimag(max(nreal, 1)))
-  if (nreal > 0) then
-    if (f_fp32) then
-      prec = SS3D_FP32
-      chunk = int(max(1_8, min(int(nreal, 8), (256_8*1024_8*1024_8)/max(ncell, 1_8))))
-      allocate(buf32(ncell, chunk))
-      do r0 = 0, nreal - 1, chunk
-        cnt = min(chunk, nreal - r0)
-        call ss3d_check(ss3d_realize(ctx, spec, int(iseed, c_int), int(igauss, c_int), int(r0, c_int64_t), &
-                                     int(cnt, c_int), int(nx, c_int), int(ny, c_int), int(nz, c_int), prec, &
-                                     c_loc(buf32), zmean(r0 + 1:), zvar(r0 + 1:), zimag(r0 + 1:)), 'realizations')
-        res(:, r0 + 1:r0 + cnt) = real(buf32(:, 1:cnt), dp)
-      end do
-      deallocate(buf32)
-    else
-      prec = SS3D_FP64
-      call ss3d_check(ss3d_realize(ctx, spec, int(iseed, c_int), int(igauss, c_int), 0_c_int64_t, int(nreal, c_int), &
-                                   int(nx, c_int), int(ny, c_int), int(nz, c_int), prec, c_loc(res), zmean, zvar, &
-                                   zimag), 'realizations')
-    end if
+  allocate(zmean(max(nreal, 1)), zvar(max(nreal, 1)), zimag(max(nreal, 1)))
+  prec = SS3D_FP64
+  if (f_fp32) prec = SS3D_FP32
+  outs%origin = org
+  outs%spacing = spc
+  if (o_bin) outs%bin_path = cstr(trim(outbase)//'.bin', p_bin)
+  if (o_vti) outs%vti_path = cstr(trim(outbase)//'.vti', p_vti)
+  if (o_vtk_all .or. o_vtk1_bin) then
+    outs%vtk_base = cstr(trim(outbase), p_vtk)
+    outs%vtk_count = int(nreal, c_int32_t)
+    if (.not. o_vtk_all) outs%vtk_count = int(min(1, nreal), c_int32_t)
   end if
+  if (o_gslib) outs%gslib_path = cstr(trim(outbase)//'.out', p_out)
+  if (o_vtk1_ascii) outs%vtk_ascii_path = cstr(trim(outbase)//'_1.vtk', p_vtka)
+  vario_ok = .false.
+  if (f_vario) call vario_read(trim(varfile))
+  call ss3d_check(ss3d_writer_open(ctx, outs, int(nx, c_int), int(ny, c_int), int(nz, c_int), int(nreal, c_int), &
+                                   prec, wr), 'output files')
+
+  ! ---- realizations
+  chunk = int(ss3d_writer_slot(wr))
+  tvario = 0
+  do r0 = 0, nreal - 1, chunk
+    cnt = min(chunk, nreal - r0)
+    call ss3d_check(ss3d_writer_realize(wr, spec, int(iseed, c_int), int(igauss, c_int), int(cnt, c_int), &
+                                        zmean(r0 + 1:), zvar(r0 + 1:), zimag(r0 + 1:), dfield), 'realizations')
+    if (vario_ok) then
+      call system_clock(tv0)
+      call ss3d_check(ss3d_variogram(ctx, dfield, 1_c_int, prec, int(nx, c_int), int(ny, c_int), int(nz, c_int), &
+                                     int(cnt, c_int), int(ndir, c_int), dirs, gam(:, :, r0 + 1:)), 'variograms')
+      call system_clock(tv1)
+      tvario = tvario + (tv1 - tv0)
+    end if
+  end do
   do i = 1, nreal
     write(*, '(a,i3,a,f9.4,a,f9.4,a,es9.2)') ' realization', i, '  mean=', zmean(i), '  variance=', zvar(i), &
       '  max|Im|=', zimag(i)
   end do
   i = ss3d_spectrum_destroy(spec)
   call system_clock(t_real)
-  write(*, '(a,f8.2,a)') ' time realizations         : ', real(t_real - t_spec)/real(rate), ' s'
+  write(*, '(a,f8.2,a)') ' time realizations         : ', real(t_real - t_spec - tvario)/real(rate), ' s'
   if (f_vario) then
     call vario_report(trim(varfile))
     call system_clock(t_vario)
-    write(*, '(a,f8.2,a)') ' time variograms           : ', real(t_vario - t_real)/real(rate), ' s'
+    write(*, '(a,f8.2,a)') ' time variograms           : ', real(t_vario - t_real + tvario)/real(rate), ' s'
   end if
-  i = ss3d_context_destroy(ctx)
 
-  ! ---- outputs (same files and order as the reference)
-  if (o_bin) call write_bin(trim(outbase)//'.bin', res, int(ncell), nreal)
-  if (o_vtk_all .or. o_vtk1_bin) then
-    do i = 1, nreal
-      if (o_vtk_all .or. i == 1) then
-        write(fname, '(a,a,i0,a)') trim(outbase), '_', i, '.vtk'
-        call write_vtk_binary(fname, res(:, i), nx, ny, nz, org, spc)
-      end if
-    end do
-  end if
-  if (o_vti) call write_vti_file(trim(outbase)//'.vti', res, nx, ny, nz, nreal, org, spc)
-  if (o_gslib) call write_gslib(trim(outbase)//'.out', res, int(ncell), nreal)
-  if (o_vtk1_ascii) call write_vtk_ascii(trim(outbase)//'_1.vtk', res(:, 1), nx, ny, nz, org, spc)
+  ! ---- outputs: wait for the streamed files, write the GSLIB/ASCII files (same files as the reference)
+  call ss3d_check(ss3d_writer_close(wr), 'output files')
+  i = ss3d_context_destroy(ctx)
 
   call system_clock(t_end)
   write(*, '(a,f8.2,a)') ' elapsed time: ', real(t_end - t0)/real(rate), ' s'
 
 contains
 
-  ! Experimental variograms of every realization along the directions of the variogram file,
-  ! their ensemble mean and sd and the model variogram: the report and the <base>_vario.csv table
-  ! of vario_report (f90:434-529), with the pair sums computed on the GPU (ss3d_variogram).
-  ! File: ndir iperreal / ndir lines sx sy sz nlag (lag k = k*(sx,sy,sz) cells).
-  subroutine vario_report(vfile)
+  ! Reads the variogram file (f90:454-462): ndir iperreal / ndir lines sx sy sz nlag (lag k =
+  ! k*(sx,sy,sz) cells).  A file that cannot be opened is reported where the reference reports it,
+  ! after the realizations.
+  subroutine vario_read(vfile)
     character(len=*), intent(in) :: vfile
-    integer :: ndir, iper, id, il, r, nlagmax, nfit, u
-    integer(c_int), allocatable :: dirs(:, :)
-    real(dp), allocatable, target :: gam(:, :, :)
-    real(dp) :: c0, gm, gsd, gth, h, rms, mx, e, hx, hy, hz
-    integer(8) :: np
-    character(len=512) :: vname
+    integer :: id, u, nlagmax
     open(newunit=u, file=vfile, status='old', action='read', iostat=ios)
-    if (ios /= 0) then
-      write(*,*) 'cannot open variogram file: ', vfile; return
-    end if
+    if (ios /= 0) return
     read(u, *) ndir, iper
     allocate(dirs(4, ndir))
     do id = 1, ndir
@@ -402,9 +275,21 @@ contains
     nlagmax = max(1, maxval(dirs(4, :)))
     allocate(gam(nlagmax, ndir, max(nreal, 1)))
     gam = 0.0_dp
-    if (nreal > 0) call ss3d_check(ss3d_variogram(ctx, c_loc(res), 0_c_int, SS3D_FP64, int(nx, c_int), &
-                                   int(ny, c_int), int(nz, c_int), int(nreal, c_int), int(ndir, c_int), &
-                                   dirs, gam), 'variograms')
+    vario_ok = .true.
+  end subroutine vario_read
+
+  ! Experimental variograms of every realization along the directions of the variogram file,
+  ! their ensemble mean and sd and the model variogram: the report and the <base>_vario.csv table
+  ! of vario_report (f90:434-529), with the pair sums computed on the GPU (ss3d_variogram).
+  subroutine vario_report(vfile)
+    character(len=*), intent(in) :: vfile
+    integer :: id, il, r, nfit, u
+    real(dp) :: c0, gm, gsd, gth, h, rms, mx, e, hx, hy, hz
+    integer(8) :: np
+    character(len=512) :: vname
+    if (.not. vario_ok) then
+      write(*,*) 'cannot open variogram file: ', vfile; return
+    end if
     c0 = nugget
     do i = 1, nst
       c0 = c0 + st(i)%sill
@@ -454,6 +339,20 @@ contains
     close(u)
     write(*, '(a,a)') ' variogram table: ', trim(vname)
   end subroutine vario_report
+
+  ! Copies s into buf as a NUL-terminated C string and returns its address.
+  function cstr(s, buf) result(p)
+    character(len=*), intent(in) :: s
+    character(kind=c_char), target, intent(inout) :: buf(:)
+    type(c_ptr) :: p
+    integer :: j
+    if (len(s) + 1 > size(buf)) stop 'output file name too long'
+    do j = 1, len(s)
+      buf(j) = s(j:j)
+    end do
+    buf(len(s) + 1) = c_null_char
+    p = c_loc(buf)
+  end function cstr
 
   function upper(s) result(u)
     character(len=*), intent(in) :: s

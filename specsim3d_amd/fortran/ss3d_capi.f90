@@ -9,10 +9,11 @@ module ss3d_capi
   use iso_c_binding
   implicit none
   private
-  public :: ss3d_structure_t, ss3d_spec_diag_t, SS3D_FP64, SS3D_FP32
+  public :: ss3d_structure_t, ss3d_spec_diag_t, ss3d_outputs_t, SS3D_FP64, SS3D_FP32
   public :: ss3d_context_create, ss3d_context_destroy, ss3d_periodic_grid, ss3d_spectrum_create, &
             ss3d_spectrum_destroy, ss3d_realize, ss3d_draws_per_realization, ss3d_check, &
-            ss3d_variogram, ss3d_npairs, ss3d_covariance
+            ss3d_variogram, ss3d_npairs, ss3d_covariance, ss3d_writer_open, ss3d_writer_slot, &
+            ss3d_writer_realize, ss3d_writer_close
 
   integer(c_int), parameter :: SS3D_FP64 = 0, SS3D_FP32 = 1
 
@@ -25,6 +26,14 @@ module ss3d_capi
 
   type, bind(C) :: ss3d_spec_diag_t
     real(c_double) :: max_imag = 0, max_real = 0, c0 = 0, spos = 0, sneg = 0
+  end type
+
+  ! ss3d_outputs: the files of one run (NUL-terminated paths, c_null_ptr = not written)
+  type, bind(C) :: ss3d_outputs_t
+    type(c_ptr) :: bin_path = c_null_ptr, vti_path = c_null_ptr, vtk_base = c_null_ptr
+    integer(c_int32_t) :: vtk_count = 0, reserved = 0
+    type(c_ptr) :: gslib_path = c_null_ptr, vtk_ascii_path = c_null_ptr
+    real(c_double) :: origin(3) = 0, spacing(3) = 1
   end type
 
   interface
@@ -99,6 +108,34 @@ module ss3d_capi
       type(ss3d_structure_t), intent(in) :: st(*)
       integer(c_int), value :: nst
       real(c_double), value :: nugget, hx, hy, hz
+    end function
+
+    ! streaming output writer (the reference's writers, f90:307-391, 1357-1414)
+    integer(c_int) function ss3d_writer_open(ctx, o, nx, ny, nz, nreal, precision, w) bind(C, name='ss3d_writer_open')
+      import :: c_int, c_ptr, ss3d_outputs_t
+      type(c_ptr), value :: ctx
+      type(ss3d_outputs_t), intent(in) :: o
+      integer(c_int), value :: nx, ny, nz, nreal, precision
+      type(c_ptr), intent(out) :: w
+    end function
+
+    integer(c_int) function ss3d_writer_slot(w) bind(C, name='ss3d_writer_slot')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: w
+    end function
+
+    integer(c_int) function ss3d_writer_realize(w, spec, iseed, igauss, nreal, mean, var, maximag, dev_field) &
+        bind(C, name='ss3d_writer_realize')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: w, spec
+      integer(c_int), value :: iseed, igauss, nreal
+      real(c_double), intent(out) :: mean(*), var(*), maximag(*)
+      type(c_ptr), intent(out) :: dev_field
+    end function
+
+    integer(c_int) function ss3d_writer_close(w) bind(C, name='ss3d_writer_close')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: w
     end function
 
     integer(c_int) function ss3d_last_error_c(buf, length) bind(C, name='ss3d_last_error')
