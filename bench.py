@@ -335,6 +335,28 @@ def writer_leg(S, eng, spec, nreal):
     return out
 
 
+def spectrum_file_leg(S, eng, spec, grid, build_ms):
+    """The workload's spectral density saved to and loaded from a file (ss3d_spectrum_save/load), timed
+    against building it on the device."""
+    wd = tempfile.mkdtemp(prefix="ss3d_spec_")
+    path = os.path.join(wd, "c2.spec")
+    try:
+        t0 = time.perf_counter()
+        spec.save(path)
+        t_save = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        ld = eng.load_spectrum(path, STRUCTS, NUGGET, 1.0, 1.0, 1.0, grid)
+        t_load = time.perf_counter() - t0
+        ld.close()
+    finally:
+        import shutil
+
+        shutil.rmtree(wd, ignore_errors=True)
+    return {"file_GB": round(size / 1e9, 3), "save_s": round(t_save, 3), "load_s": round(t_load, 3),
+            "build_on_device_ms": round(build_ms, 2)}
+
+
 def load_traffic(kind):
     """HBM bytes per launch of a pass kind measured with rocprofv3 counters (profiles/traffic_c2.json:
     FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB device copy)."""
@@ -427,6 +449,7 @@ def main():
         hip.free(out64)
     hip.free(stats)
     writer = None if args.no_writer else writer_leg(S, eng, spec, nreal)
+    specfile = None if args.no_writer else spectrum_file_leg(S, eng, spec, grid, spec_ms)
     spec.close()
 
     uneven = None if args.no_batch else uneven_batch(S, eng, 3)
@@ -473,6 +496,7 @@ def main():
         "uneven_batch": uneven,
         "large_job": large,
         "output_files": writer,
+        "spectrum_file": specfile,
         "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
                         "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -94,6 +94,18 @@ int64_t ss3d_draws_per_realization(int npx, int npy, int npz, int igauss);
 /* Park-Miller state that realization r (0-based) starts from, seeded as f90:1302-1303. */
 int64_t ss3d_realization_state(int iseed, int npx, int npy, int npz, int igauss, int64_t r);
 
+/* The spectral density as a file (no counterpart in the reference, which recomputes it every run):
+ * the FP64 amplitudes, the FP32 half spectrum, the diagnostics and the key (structures, nugget,
+ * spacing, periodic grid) of a spectrum made by ss3d_spectrum_create.  ss3d_spectrum_load makes a
+ * spectrum of ctx from such a file; with st != NULL the file must have been made for exactly that
+ * model, spacing and grid (SS3D_EINVAL otherwise); st = NULL takes the file's key.  The payload is
+ * checked against the diagnostics (sum of amp^2 = positive spectral mass).  Realizations of a loaded
+ * spectrum are identical to those of the spectrum that was saved. */
+int ss3d_spectrum_save(const ss3d_spectrum* spec, const char* path);
+int ss3d_spectrum_load(ss3d_context* ctx, const char* path, const ss3d_structure* st, int nst, double nugget,
+                       double dx, double dy, double dz, int npx, int npy, int npz, ss3d_spectrum** out,
+                       ss3d_spec_diag* diag);
+
 /* Realizations r0 .. r0+nreal-1 of the stream seeded by iseed (replaces f90:1305-1344):
  * Hermitian coefficient fill, inverse transform, crop of the first nx*ny*nz corner.
  * host_out: nx*ny*nz*nreal values, double for SS3D_FP64 and float for SS3D_FP32.

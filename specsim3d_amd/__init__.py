@@ -127,6 +127,11 @@ SIGNATURES = {
                                            ctypes.POINTER(_vp)]),
     "ss3d_writer_put": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int]),
     "ss3d_writer_close": (ctypes.c_int, [_vp]),
+    "ss3d_spectrum_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "ss3d_spectrum_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(Structure), ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
+                                          ctypes.POINTER(SpecDiag)]),
     "ss3d_format_es": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
 }
 
@@ -206,6 +211,10 @@ class Spectrum:
     def nbytes(self):
         return lib().ss3d_spectrum_bytes(self.handle)
 
+    def save(self, path):
+        """Write the spectrum, its diagnostics and its model key to a file (ss3d_spectrum_save)."""
+        check(lib().ss3d_spectrum_save(self.handle, os.fsencode(path)))
+
     def close(self):
         if self.handle:
             lib().ss3d_spectrum_destroy(self.handle)
@@ -234,6 +243,24 @@ class Engine:
                                          ctypes.byref(d)))
         diag = dict(max_imag=d.max_imag, max_real=d.max_real, c0=d.c0, spos=d.spos, sneg=d.sneg)
         sp = Spectrum(self, h, (npx, npy, npz), diag)
+        self._deps.add(sp)
+        return sp
+
+    def load_spectrum(self, path, sts=None, nugget=0.0, dx=1.0, dy=1.0, dz=1.0, grid=(1, 1, 1)):
+        """Spectrum from a file of Spectrum.save; with sts given, the file must match that model, spacing
+        and periodic grid."""
+        h = _vp()
+        d = SpecDiag()
+        if sts is None:
+            arr, n = None, 0
+        else:
+            arr, n = structures(sts)
+        check(lib().ss3d_spectrum_load(self.handle, os.fsencode(path), arr, n, nugget, dx, dy, dz, *grid,
+                                       ctypes.byref(h), ctypes.byref(d)))
+        with open(path, "rb") as fh:  # header: 8-byte magic, then int32 version, nst, npx, npy, npz, hxa
+            grid = tuple(int(v) for v in np.frombuffer(fh.read(32)[8:], dtype="<i4")[2:5])
+        diag = dict(max_imag=d.max_imag, max_real=d.max_real, c0=d.c0, spos=d.spos, sneg=d.sneg)
+        sp = Spectrum(self, h, grid, diag)
         self._deps.add(sp)
         return sp
 

@@ -262,6 +262,10 @@ struct ss3d_spectrum {
     DevBuf amp64, amp32h, diagd;
     RngTables rng[2];
     ss3d_spec_diag diag{};
+    // the model this spectrum was made for (ss3d_spectrum_create / load): the key of its file
+    int key_nst = 0;
+    ss3d_structure key_st[kMaxStruct] = {};
+    double key_nugget = 0, key_d[3] = {0, 0, 0};
 };
 
 namespace {
@@ -718,6 +722,12 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     SS3D_TRY(spectrum_build(c, c->main, st, nst, nugget, dx, dy, dz, npx, npy, npz, sp));
     SS3D_TRY(spectrum_read_diag(sp.get(), c->main.stream));
     if (diag) *diag = sp->diag;
+    sp->key_nst = nst;
+    std::memcpy(sp->key_st, st, sizeof(ss3d_structure) * nst);
+    sp->key_nugget = nugget;
+    sp->key_d[0] = dx;
+    sp->key_d[1] = dy;
+    sp->key_d[2] = dz;
     c->spectra.insert(sp.get());
     *out = sp.release();
     return SS3D_OK;
@@ -1197,3 +1207,4 @@ int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
 #include "ss3d_batch.inc"
 #include "ss3d_vario.inc"
 #include "ss3d_output.inc"
+#include "ss3d_specfile.inc"
