@@ -79,6 +79,33 @@ int     or_simulate(const or_structure* st, int nst, double nugget, double dx, d
 int64_t or_npair_count(int nx, int ny, int nz, int lx, int ly, int lz);
 double  or_vario_lag(const double* f, int nx, int ny, int nz, int lx, int ly, int lz);
 
+/* ---- conditioning (cond_mod, f90:548-1083), oracle_cond.c ---- */
+/* The conditioning block without its data file name (f90:731-747). */
+typedef struct or_cond_params {
+    int32_t ixc, iyc, izc, ivc, iwc;   /* GSLIB columns, 1-based; izc, iwc may be 0 */
+    int32_t itrans, ltail, utail, ndmax, pad_;
+    double tmin, tmax, skmean, zlo, zhi, ltpar, utpar;
+    double rmax, rmed, rvert, sa1, sa2, sa3;
+} or_cond_params;
+
+/* What the reference prints about the conditioning (f90:818-819, 890, 927-931). */
+typedef struct or_cond_info {
+    int32_t nd, nmerged, nout, nsbx, nsby, nsbz, nnone, nexact, nfail, nt;
+    int64_t nsum;
+} or_cond_info;
+
+double  or_gcum(double x);
+double  or_gauinv(double p);
+void    or_heapsort_idx(const double* key, int* idx /* 1-based entries */, int n);
+void    or_nscore(int nd, const double* v, const double* w, double* g, double* vr, double* gt);
+double  or_backtr(double y, int nt, const double* vr, const double* gt, double zmin, double zmax, int lt,
+                  double ltp, int ut, double utp);
+/* condition_field without its file I/O: returns 0, or 1 (ndmax), 2 (radii), 3 (no data in grid). */
+int     or_condition(double* res, int nx, int ny, int nz, int nreal, double xmin, double ymin, double zmin,
+                     double dx, double dy, double dz, const or_structure* st, int nst, double nugget,
+                     const or_cond_params* p, const double* rows, int nrow, int ncol, or_cond_info* info,
+                     int* ix, int* iy, int* iz, double* dval, double* dg, double* vrt, double* gtt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,20 @@ class SpecDiag(ctypes.Structure):
                 ("spos", ctypes.c_double), ("sneg", ctypes.c_double)]
 
 
+class CondParams(ctypes.Structure):
+    """The conditioning block without its data file name (src/specsim3d.f90:731-747)."""
+    _fields_ = [("ixc", ctypes.c_int32), ("iyc", ctypes.c_int32), ("izc", ctypes.c_int32), ("ivc", ctypes.c_int32),
+                ("iwc", ctypes.c_int32), ("itrans", ctypes.c_int32), ("ltail", ctypes.c_int32),
+                ("utail", ctypes.c_int32), ("ndmax", ctypes.c_int32), ("pad_", ctypes.c_int32)] + \
+               [(n, ctypes.c_double) for n in ("tmin", "tmax", "skmean", "zlo", "zhi", "ltpar", "utpar", "rmax",
+                                                "rmed", "rvert", "sa1", "sa2", "sa3")]
+
+
+class CondInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("nd", "nmerged", "nout", "nsbx", "nsby", "nsbz", "nnone", "nexact",
+                                               "nfail", "nt")] + [("nsum", ctypes.c_int64)]
+
+
 _lib = None
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -70,6 +84,22 @@ def lib():
         L.or_npair_count.argtypes = [ctypes.c_int] * 6
         L.or_vario_lag.restype = ctypes.c_double
         L.or_vario_lag.argtypes = [_dp] + [ctypes.c_int] * 6
+        L.or_gcum.restype = ctypes.c_double
+        L.or_gcum.argtypes = [ctypes.c_double]
+        L.or_gauinv.restype = ctypes.c_double
+        L.or_gauinv.argtypes = [ctypes.c_double]
+        L.or_heapsort_idx.argtypes = [_dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.or_nscore.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.or_backtr.restype = ctypes.c_double
+        L.or_backtr.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.or_condition.restype = ctypes.c_int
+        L.or_condition.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double,
+                                   ctypes.POINTER(CondParams), _dp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(CondInfo), _ip, _ip, _ip, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -191,3 +221,57 @@ def vario_lag(f, lx, ly, lz):
     f = np.ascontiguousarray(f, dtype=np.float64)
     nz, ny, nx = f.shape
     return lib().or_vario_lag(_p(f), nx, ny, nz, lx, ly, lz)
+
+
+# ---- conditioning (src/specsim3d.f90:548-1083)
+def gcum(x):
+    return lib().or_gcum(float(x))
+
+
+def gauinv(p):
+    return lib().or_gauinv(float(p))
+
+
+def heapsort_idx(key):
+    """1-based permutation that sorts key ascending, in the reference's heapsort order."""
+    key = np.ascontiguousarray(key, dtype=np.float64)
+    idx = (ctypes.c_int * max(key.size, 1))(*range(1, key.size + 1))
+    lib().or_heapsort_idx(_p(key), idx, key.size)
+    return np.array(idx[:key.size])
+
+
+def nscore(v, w):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    g, vr, gt = np.zeros(v.size), np.zeros(v.size), np.zeros(v.size)
+    lib().or_nscore(v.size, _p(v), _p(w), _p(g), _p(vr), _p(gt))
+    return g, vr, gt
+
+
+def cond_params(**kw):
+    p = CondParams()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def condition(res, origin, spacing, sts, nugget, params, rows):
+    """Conditions res (nreal, nz, ny, nx) FP64 in place; returns a dict with the used data nodes,
+    their values, Gaussian values, the normal-score table and the counts the reference prints."""
+    nreal, nz, ny, nx = res.shape
+    assert res.dtype == np.float64 and res.flags.c_contiguous
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    nrow, ncol = rows.shape
+    info = CondInfo()
+    n = max(nrow, 1)
+    ix, iy, iz = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    dval, dg, vrt, gtt = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc = lib().or_condition(_p(res), nx, ny, nz, nreal, *map(float, origin), *map(float, spacing), structures(sts),
+                            len(sts), float(nugget), ctypes.byref(params), _p(rows), nrow, ncol, ctypes.byref(info),
+                            ix, iy, iz, _p(dval), _p(dg), _p(vrt), _p(gtt))
+    if rc != 0:
+        raise ValueError(f"or_condition failed ({rc})")
+    nd = info.nd
+    return dict(rc=rc, info={f[0]: getattr(info, f[0]) for f in CondInfo._fields_},
+                ijk=np.array([ix[:nd], iy[:nd], iz[:nd]]).T, dval=dval[:nd], dg=dg[:nd], vrt=vrt[:info.nt],
+                gtt=gtt[:info.nt])

@@ -1,0 +1,136 @@
+"""The oracle's conditioning (oracle/oracle_cond.c, a restatement of cond_mod,
+src/specsim3d.f90:548-1083) pinned on the CPU:
+
+  - against the reference executable's own conditional simulations (tests/golden/cond, FP64
+    realizations written by the reference build): conditioned fields, the counts it prints, the
+    used data (<base>_conddata.dat) and the normal-score table (<base>_nscore.trn);
+  - against mathematics that fixes the result: the data are reproduced at their nodes; with every
+    datum in the neighbourhood the kriged residual equals exact simple kriging solved by numpy;
+    the normal scores of equal-weight data are the standard normal quantiles (scipy); gauinv and
+    gcum against scipy's normal ppf/cdf; back transform of a table value is that value; the
+    heapsort permutation sorts its keys.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.stats import norm
+
+import golden_cases as G
+import oracle_ffi as O
+
+CASES = ["sk3d", "ns2d", "rot3d", "inf"]
+
+
+def run_case(name):
+    c = G.cond_case(name)
+    res = O.simulate(c["sts"], c["nugget"], *c["spacing"], *c["grid"], c["ipad"], c["iseed"], c["nreal"],
+                     c["igauss"])["res"].copy()
+    unc = res.copy()
+    out = O.condition(res, c["origin"], c["spacing"], c["sts"], c["nugget"], O.cond_params(**c["params"]), c["rows"])
+    return c, unc, res, out
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_conditioned_fields_equal_reference(name):
+    c, unc, res, out = run_case(name)
+    ref = G.cond_field(name)
+    assert res.shape == ref.shape
+    assert np.array_equal(res, ref), np.abs(res - ref).max()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_counts_and_files_equal_reference(name):
+    c, unc, res, out = run_case(name)
+    info = out["info"]
+    lines = G.cond_manifest()[name]["stdout"]
+    cd = [ln for ln in lines if "conditioning data:" in ln][0].replace("(", " ").replace(",", " ").split()
+    assert (info["nd"], info["nmerged"], info["nout"]) == (int(cd[2]), int(cd[5]), int(cd[7]))
+    sb = [ln for ln in lines if "super blocks:" in ln][0].split()
+    assert (info["nsbx"], info["nsby"], info["nsbz"]) == (int(sb[2]), int(sb[3]), int(sb[4]))
+    mn = re.search(r"mean neighbours per node:\s+([\d.]+)\s+\(nodes without data: (\d+), data nodes: (\d+)\)",
+                   [ln for ln in lines if "mean neighbours" in ln][0])
+    nx, ny, nz = c["grid"]
+    assert "%6.1f" % (info["nsum"] / (nx * ny * nz)) == "%6.1f" % float(mn.group(1))
+    assert info["nnone"] == int(mn.group(2)) and info["nexact"] == int(mn.group(3))
+    assert not any("WARNING" in ln for ln in lines) and info["nfail"] == 0
+    # <base>_conddata.dat: x y z value gauss of every used node (es17.8)
+    with open(os.path.join(G.COND, f"{c['base']}_conddata.dat")) as fh:
+        body = np.array([list(map(float, ln.split())) for ln in fh.read().splitlines()[7:]])
+    ijk = out["ijk"]
+    xyz = np.array(c["origin"]) + (ijk - 1) * np.array(c["spacing"])
+    assert np.allclose(body[:, :3], xyz, rtol=1e-8, atol=1e-12)
+    assert np.allclose(body[:, 3], out["dval"], rtol=1e-8, atol=0) and np.allclose(body[:, 4], out["dg"], rtol=1e-8,
+                                                                                    atol=1e-12)
+    if c["params"]["itrans"] == 1:
+        with open(os.path.join(G.COND, f"{c['base']}_nscore.trn")) as fh:
+            tab = np.array([list(map(float, ln.split())) for ln in fh.read().splitlines()[4:]])
+        assert np.allclose(tab[:, 0], out["vrt"], rtol=1e-8) and np.allclose(tab[:, 1], out["gtt"], rtol=1e-8,
+                                                                              atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["sk3d", "rot3d"])
+def test_data_reproduced(name):
+    c, unc, res, out = run_case(name)
+    i, j, k = (out["ijk"] - 1).T
+    want = out["dg"] + c["params"]["skmean"]
+    assert np.abs(res[:, k, j, i] - want[None]).max() < 1e-12
+
+
+def test_simple_kriging_exact():
+    """Every datum in the neighbourhood: conditioned = unconditional + SK(data - unconditional at data)."""
+    c, unc, res, out = run_case("sk3d")
+    st = c["sts"][0]
+    rot, _ = O.setrot(st)
+    ijk = out["ijk"] - 1
+    xyz = np.array(c["origin"]) + ijk * np.array(c["spacing"])
+
+    def cov(d):
+        r = d @ rot.T
+        h = np.sqrt((r ** 2).sum(-1)) / st[2]
+        return np.where(h < 1, st[1] * (1 - 1.5 * h + 0.5 * h ** 3), 0.0)
+
+    K = cov(xyz[:, None] - xyz[None]) + 1e-10 * np.eye(len(xyz))
+    nx, ny, nz = c["grid"]
+    g = np.meshgrid(*(np.arange(n, dtype=float) for n in (nx, ny, nz)), indexing="ij")
+    P = np.c_[g[0].ravel(order="F"), g[1].ravel(order="F"), g[2].ravel(order="F")] * np.array(c["spacing"]) + \
+        np.array(c["origin"])
+    W = np.linalg.solve(K, cov(xyz[None] - P[:, None]).T).T
+    for r in range(c["nreal"]):
+        u = unc[r].ravel()
+        resid = out["dg"] - u[ijk[:, 0] + nx * (ijk[:, 1] + ny * ijk[:, 2])]
+        want = (u + W @ resid)
+        at = ijk[:, 0] + nx * (ijk[:, 1] + ny * ijk[:, 2])
+        want[at] = out["dg"]
+        assert np.abs(res[r].ravel() - want).max() < 1e-8
+
+
+def test_normal_scores_and_back_transform():
+    rng = np.random.default_rng(1)
+    v = np.exp(rng.normal(size=300))
+    g, vr, gt = O.nscore(v, np.ones(300))
+    assert np.array_equal(vr, np.sort(v))
+    assert np.abs(gt - norm.ppf((np.arange(300) + 0.5) / 300)).max() < 1e-12
+    assert np.array_equal(g[np.argsort(v)], gt)
+    for p in [1e-20, 1e-9, 0.01, 0.02425, 0.3, 0.5, 0.9, 0.97575, 0.999]:
+        pc = max(p, 1e-15)
+        assert abs(O.gauinv(p) - norm.ppf(pc)) < 1e-12 * max(1, abs(norm.ppf(pc)))
+    # near 1 the Halley step evaluates gcum(x) - p with cancellation (the reference's algorithm):
+    # about 1e-9 relative at 1 - 1e-12
+    assert abs(O.gauinv(1 - 1e-12) - norm.ppf(1 - 1e-12)) < 1e-7 * norm.ppf(1 - 1e-12)
+    for x in [-8, -2, -0.3, 0, 0.7, 3, 9]:
+        assert abs(O.gcum(x) - norm.cdf(x)) < 1e-15
+    L = O.lib()
+    vr_, gt_ = np.ascontiguousarray(vr), np.ascontiguousarray(gt)
+    for q in (0, 17, 299):
+        z = L.or_backtr(gt[q], 300, O._p(vr_), O._p(gt_), 0.0, 100.0, 1, 1.0, 1, 1.0)
+        assert z == pytest.approx(vr[q], rel=1e-15)
+
+
+def test_heapsort_permutation():
+    rng = np.random.default_rng(2)
+    key = rng.integers(0, 20, 101).astype(float)
+    idx = O.heapsort_idx(key)
+    assert sorted(idx.tolist()) == list(range(1, 102))
+    assert np.all(np.diff(key[idx - 1]) >= 0)
