@@ -242,6 +242,55 @@ int ss3d_writer_close(ss3d_writer* w);
  * correctly rounded digits (ties to even), E+dd exponents, +ddd beyond 99, '*' fill on overflow. */
 int ss3d_format_es(const double* v, int64_t n, int w, int d, char* out);
 
+/* ---------------------------------------------------------------- conditioning
+ * Conditioning of realizations on data by simple kriging of their residual (module cond_mod,
+ * src/specsim3d.f90:548-1083): z_c = z*_d + z_s - z*_s at every node, from the ndmax nearest data
+ * in the anisotropic search ellipsoid, then the normal-score back transform (itrans = 1) or the
+ * simple-kriging mean added (f90:910-925).  The caller prepares the data as the reference does
+ * (f90:760-846: snapped to nodes, merged, normal scores or value - skmean) and passes them as 1-based
+ * node indices (i, j, k) and Gaussian values.  ss3d_cond_create computes the kriging weights of
+ * every node once (neighbour search, Cholesky solve with the reference's ridge rule);
+ * ss3d_cond_apply conditions nreal realizations in device memory on the context's stream. */
+typedef struct ss3d_cond_params {
+    double rmax, rmed, rvert; /* search radii (f90:741) */
+    double sa1, sa2, sa3;     /* search angles (f90:742) */
+    int32_t ndmax;            /* nearest data per node, 1..100 (f90:743) */
+    int32_t itrans;           /* 1: back transform with the table below; 0: add skmean */
+    double skmean;            /* simple-kriging mean (itrans = 0) */
+    int32_t nt;               /* normal-score table: vr (values) and gt (scores), ascending */
+    int32_t ltail, utail;     /* tail options of the back transform (f90:736-737) */
+    int32_t reserved;         /* 0 */
+    const double* vr;
+    const double* gt;
+    double zlo, zhi, ltpar, utpar;
+} ss3d_cond_params;
+
+/* What the reference prints about the kriging (f90:890, 927-931), and the time spent. */
+typedef struct ss3d_cond_info {
+    int32_t nsbx, nsby, nsbz; /* super blocks */
+    int32_t nnone;            /* nodes without data in the search ellipsoid */
+    int32_t nexact;           /* nodes holding a datum */
+    int32_t nfail;            /* systems that needed a larger ridge, or were not solved */
+    int64_t nsum;             /* neighbours summed over nodes */
+    double weights_s;         /* ss3d_cond_create, host clock */
+    double apply_s;           /* kriging of realizations: device time of every ss3d_cond_apply */
+    double backtr_s;          /* back transform / mean: device time */
+} ss3d_cond_info;
+
+typedef struct ss3d_cond ss3d_cond;
+int ss3d_cond_create(ss3d_context* ctx, const ss3d_structure* st, int nst, double nugget, int nx, int ny, int nz,
+                     double xmin, double ymin, double zmin, double dx, double dy, double dz, int nd,
+                     const int32_t* ijk, const double* g, const ss3d_cond_params* p, ss3d_cond** out,
+                     ss3d_cond_info* info);
+/* dev_field: nreal realizations (nx*ny*nz values each, x fastest) of the given precision. */
+int ss3d_cond_apply(ss3d_cond* cond, void* dev_field, int precision, int nreal);
+/* Counters and accumulated times (waits for the context's stream). */
+int ss3d_cond_info_get(ss3d_cond* cond, ss3d_cond_info* info);
+int ss3d_cond_destroy(ss3d_cond* cond);
+/* Conditions every later ss3d_writer_realize chunk with cond (same context) before it is written,
+ * as the reference conditions res before its writers (f90:1347). */
+int ss3d_writer_condition(ss3d_writer* w, ss3d_cond* cond);
+
 /* ---------------------------------------------------------------- variograms
  * Experimental semivariograms, replacing vario_lag (src/specsim3d.f90:405-431) as vario_report
  * calls it (f90:462-483).  For realization r, direction d with step (sx, sy, sz) and lag

@@ -69,6 +69,23 @@ class Outputs(ctypes.Structure):
                 ("spacing", ctypes.c_double * 3)]
 
 
+class CondParams(ctypes.Structure):
+    """ss3d_cond_params: search ellipsoid, neighbours and back transform (src/specsim3d.f90:731-747)."""
+    _fields_ = [("rmax", ctypes.c_double), ("rmed", ctypes.c_double), ("rvert", ctypes.c_double),
+                ("sa1", ctypes.c_double), ("sa2", ctypes.c_double), ("sa3", ctypes.c_double),
+                ("ndmax", ctypes.c_int32), ("itrans", ctypes.c_int32), ("skmean", ctypes.c_double),
+                ("nt", ctypes.c_int32), ("ltail", ctypes.c_int32), ("utail", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("vr", ctypes.c_void_p), ("gt", ctypes.c_void_p),
+                ("zlo", ctypes.c_double), ("zhi", ctypes.c_double), ("ltpar", ctypes.c_double),
+                ("utpar", ctypes.c_double)]
+
+
+class CondInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("nsbx", "nsby", "nsbz", "nnone", "nexact", "nfail")] + \
+               [("nsum", ctypes.c_int64), ("weights_s", ctypes.c_double), ("apply_s", ctypes.c_double),
+                ("backtr_s", ctypes.c_double)]
+
+
 _lib = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -132,6 +149,14 @@ SIGNATURES = {
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                           ctypes.POINTER(SpecDiag)]),
+    "ss3d_cond_create": (ctypes.c_int, [_vp, ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double] +
+                         [ctypes.c_int] * 3 + [ctypes.c_double] * 6 +
+                         [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CondParams),
+                          ctypes.POINTER(_vp), ctypes.POINTER(CondInfo)]),
+    "ss3d_cond_apply": (ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "ss3d_cond_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(CondInfo)]),
+    "ss3d_cond_destroy": (ctypes.c_int, [_vp]),
+    "ss3d_writer_condition": (ctypes.c_int, [_vp, _vp]),
     "ss3d_format_es": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
 }
 
@@ -264,6 +289,14 @@ class Engine:
         self._deps.add(sp)
         return sp
 
+    def conditioning(self, sts, nugget, grid, origin, spacing, ijk, g, rmax, rmed, rvert, sa=(0.0, 0.0, 0.0),
+                     ndmax=16, itrans=0, skmean=0.0, table=None, zlo=0.0, zhi=0.0, ltail=1, ltpar=1.0, utail=1,
+                     utpar=1.0):
+        """Kriging weights of every node for conditioning on data (ss3d_cond_create).  ijk: (nd, 3)
+        1-based nodes; g: Gaussian values; table = (vr, gt) for the back transform (itrans = 1)."""
+        return Conditioning(self, sts, nugget, grid, origin, spacing, ijk, g, rmax, rmed, rvert, sa, ndmax, itrans,
+                            skmean, table, zlo, zhi, ltail, ltpar, utail, utpar)
+
     def realize(self, spec, iseed, igauss, r0, nreal, nx, ny, nz, precision=FP64):
         """Realizations r0..r0+nreal-1 -> (field[nreal, nz, ny, nx], mean, var, maximag)."""
         dt = np.float64 if precision == FP64 else np.float32
@@ -350,6 +383,46 @@ class Engine:
         self.close()
 
 
+class Conditioning:
+    """Conditioning of realizations on data (ss3d_cond_*): see Engine.conditioning."""
+
+    def __init__(self, engine, sts, nugget, grid, origin, spacing, ijk, g, rmax, rmed, rvert, sa, ndmax, itrans,
+                 skmean, table, zlo, zhi, ltail, ltpar, utail, utpar):
+        self.engine = engine
+        self.grid = tuple(grid)
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        self._table = None
+        p = CondParams(rmax=rmax, rmed=rmed, rvert=rvert, sa1=sa[0], sa2=sa[1], sa3=sa[2], ndmax=ndmax,
+                       itrans=itrans, skmean=skmean, ltail=ltail, utail=utail, zlo=zlo, zhi=zhi, ltpar=ltpar,
+                       utpar=utpar)
+        if itrans == 1:
+            vr, gt = (np.ascontiguousarray(t, dtype=np.float64) for t in table)
+            self._table = (vr, gt)
+            p.nt, p.vr, p.gt = vr.size, vr.ctypes.data, gt.ctypes.data
+        arr, n = structures(sts)
+        self.handle = _vp()
+        info = CondInfo()
+        check(lib().ss3d_cond_create(engine.handle, arr, n, nugget, *grid, *origin, *spacing, len(g),
+                                     ijk.ctypes.data, g.ctypes.data, ctypes.byref(p), ctypes.byref(self.handle),
+                                     ctypes.byref(info)))
+        engine._deps.add(self)
+
+    def apply(self, dev_field, precision, nreal):
+        """Conditions nreal realizations at device address dev_field in place (context stream)."""
+        check(lib().ss3d_cond_apply(self.handle, _vp(dev_field), precision, nreal))
+
+    def info(self):
+        i = CondInfo()
+        check(lib().ss3d_cond_info_get(self.handle, ctypes.byref(i)))
+        return {f[0]: getattr(i, f[0]) for f in CondInfo._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().ss3d_cond_destroy(self.handle)
+            self.handle = None
+
+
 class Writer:
     """Streaming writer of the reference's output files (ss3d_writer_*).
 
@@ -378,6 +451,11 @@ class Writer:
         check(lib().ss3d_writer_realize(self.handle, spec.handle, iseed, igauss, count, mean.ctypes.data_as(_dp),
                                         var.ctypes.data_as(_dp), mi.ctypes.data_as(_dp), ctypes.byref(dev)))
         return mean, var, mi, dev.value
+
+    def condition(self, cond):
+        """Condition every later realize() chunk with a Conditioning before it is written."""
+        check(lib().ss3d_writer_condition(self.handle, cond.handle if cond else None))
+        self._cond = cond
 
     def put(self, field):
         """Next realizations from a host array (nreal, nz, ny, nx) of the writer's precision."""

@@ -1208,3 +1208,4 @@ int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
 #include "ss3d_vario.inc"
 #include "ss3d_output.inc"
 #include "ss3d_specfile.inc"
+#include "ss3d_cond.inc"

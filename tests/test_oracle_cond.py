@@ -134,3 +134,50 @@ def test_heapsort_permutation():
     idx = O.heapsort_idx(key)
     assert sorted(idx.tolist()) == list(range(1, 102))
     assert np.all(np.diff(key[idx - 1]) >= 0)
+
+
+@pytest.mark.parametrize("case", ["escalated", "unsolved"])
+def test_ridge_rule(case):
+    """Negative nuggets make the kriging matrix indefinite: just below zero the reference's rule
+    (ridge 1e-10 C(0), then 1e3 and 1e6 times larger, f90:989-1000) solves with the second ridge,
+    which numpy reproduces; far below, no ridge solves and the nodes keep z_s."""
+    rng = np.random.default_rng(9)
+    nx, ny, nz = 12, 10, 4
+    idx = rng.choice(nx * ny * nz, 14, replace=False)
+    i, j, k = np.unravel_index(idx, (nx, ny, nz))
+    v = rng.normal(size=14)
+    rows = np.c_[i, j, k, v].astype(float)
+    sts = [(1, 1.0, 30.0, 30.0, 30.0, 0.0, 0.0, 0.0)]
+    rot, _ = O.setrot(sts[0])
+    xyz = np.c_[i, j, k].astype(float)
+
+    def cov(d):
+        h = np.sqrt(((d @ rot.T) ** 2).sum(-1)) / 30.0
+        return np.where(h < 1, 1 - 1.5 * h + 0.5 * h ** 3, 0.0)
+
+    K = cov(xyz[:, None] - xyz[None])
+    nug = -np.linalg.eigvalsh(K).min() - 5e-9 if case == "escalated" else -0.5
+    params = dict(ixc=1, iyc=2, izc=3, ivc=4, iwc=0, tmin=-1e21, tmax=1e21, itrans=0, skmean=0.0, zlo=-5, zhi=5,
+                  ltail=1, ltpar=0.0, utail=1, utpar=0.0, rmax=100.0, rmed=100.0, rvert=100.0, sa1=0.0, sa2=0.0,
+                  sa3=0.0, ndmax=100)
+    f = np.random.default_rng(1).standard_normal((2, nz, ny, nx))
+    res = f.copy()
+    out = O.condition(res, (0, 0, 0), (1, 1, 1), sts, nug, O.cond_params(**params), rows)
+    assert out["info"]["nfail"] == nx * ny * nz - 14 and out["info"]["nexact"] == 14
+    at = i + nx * (j + ny * k)
+    mask = np.ones(nx * ny * nz, bool)
+    mask[at] = False
+    if case == "unsolved":
+        assert np.array_equal(res.reshape(2, -1)[:, mask], f.reshape(2, -1)[:, mask])
+    else:
+        c0 = 1.0 + nug
+        A = K + (nug + 1e-7 * c0) * np.eye(14)
+        g = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+        P = np.c_[g[0].ravel(order="F"), g[1].ravel(order="F"), g[2].ravel(order="F")].astype(float)
+        W = np.linalg.solve(A, cov(xyz[None] - P[:, None]).T).T
+        for r in range(2):
+            u = f[r].ravel()
+            want = u + W @ (v - u[at])
+            # the 1e3-fold ridge leaves the system nearly singular (weights ~1e5): relative check
+            assert np.abs(res[r].ravel()[mask] - want[mask]).max() < 1e-7 * np.abs(want[mask]).max()
+    assert np.abs(res.reshape(2, -1)[:, at] - v[None]).max() < 1e-15
