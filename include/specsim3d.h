@@ -277,6 +277,24 @@ typedef struct ss3d_cond_info {
     double backtr_s;          /* back transform / mean: device time */
 } ss3d_cond_info;
 
+/* The data-file part of the conditioning block (f90:732-734). */
+typedef struct ss3d_cond_columns {
+    int32_t ixc, iyc, izc, ivc, iwc; /* 1-based columns of x, y, z, value, weight; izc, iwc may be 0 */
+    int32_t itrans;                  /* 1: normal-score transform of the data */
+    double tmin, tmax;               /* trimming limits: values outside [tmin, tmax) are ignored */
+    double skmean;                   /* simple-kriging mean (itrans = 0) */
+} ss3d_cond_columns;
+
+/* The data preparation of the reference (f90:760-846): rows (nrow records of ncol values, as read
+ * from the GSLIB data file) trimmed, snapped to the nearest node, merged into the weighted mean of
+ * each node (in the reference's heap-sort order), and transformed: normal scores (itrans = 1, with
+ * the table vr/gt, nd entries, ascending) or value - skmean.  Outputs sized nrow: ijk (3 per datum,
+ * 1-based), dval (merged values), dg (Gaussian values); nd, the number merged away and the number
+ * outside the grid.  SS3D_EINVAL when no datum is inside the grid. */
+int ss3d_cond_prepare(const double* rows, int nrow, int ncol, const ss3d_cond_columns* cols, int nx, int ny, int nz,
+                      double xmin, double ymin, double zmin, double dx, double dy, double dz, int32_t* nd,
+                      int32_t* nmerged, int32_t* nout, int32_t* ijk, double* dval, double* dg, double* vr, double* gt);
+
 typedef struct ss3d_cond ss3d_cond;
 int ss3d_cond_create(ss3d_context* ctx, const ss3d_structure* st, int nst, double nugget, int nx, int ny, int nz,
                      double xmin, double ymin, double zmin, double dx, double dy, double dz, int nd,

@@ -80,6 +80,12 @@ class CondParams(ctypes.Structure):
                 ("utpar", ctypes.c_double)]
 
 
+class CondColumns(ctypes.Structure):
+    """ss3d_cond_columns: the data-file part of the conditioning block (src/specsim3d.f90:732-734)."""
+    _fields_ = [(n, ctypes.c_int32) for n in ("ixc", "iyc", "izc", "ivc", "iwc", "itrans")] + \
+               [("tmin", ctypes.c_double), ("tmax", ctypes.c_double), ("skmean", ctypes.c_double)]
+
+
 class CondInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("nsbx", "nsby", "nsbz", "nnone", "nexact", "nfail")] + \
                [("nsum", ctypes.c_int64), ("weights_s", ctypes.c_double), ("apply_s", ctypes.c_double),
@@ -149,6 +155,9 @@ SIGNATURES = {
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                           ctypes.POINTER(SpecDiag)]),
+    "ss3d_cond_prepare": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(CondColumns)] +
+                          [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_int32)] * 3 +
+                          [ctypes.c_void_p] * 5),
     "ss3d_cond_create": (ctypes.c_int, [_vp, ctypes.POINTER(Structure), ctypes.c_int, ctypes.c_double] +
                          [ctypes.c_int] * 3 + [ctypes.c_double] * 6 +
                          [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CondParams),
@@ -381,6 +390,25 @@ class Engine:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def cond_prepare(rows, grid, origin, spacing, ixc, iyc, izc, ivc, iwc=0, tmin=-1e21, tmax=1e21, itrans=0,
+                 skmean=0.0):
+    """The reference's data preparation (ss3d_cond_prepare): data rows -> dict(ijk (nd, 3) 1-based,
+    dval, dg, vr, gt (itrans = 1), nd, nmerged, nout)."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    nrow, ncol = rows.shape
+    n = max(nrow, 1)
+    ijk = np.zeros((n, 3), dtype=np.int32)
+    dval, dg, vr, gt = (np.zeros(n) for _ in range(4))
+    nd, nm, no = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    cc = CondColumns(ixc, iyc, izc, ivc, iwc, itrans, tmin, tmax, skmean)
+    check(lib().ss3d_cond_prepare(rows.ctypes.data, nrow, ncol, ctypes.byref(cc), *grid, *origin, *spacing,
+                                  ctypes.byref(nd), ctypes.byref(nm), ctypes.byref(no), ijk.ctypes.data,
+                                  dval.ctypes.data, dg.ctypes.data, vr.ctypes.data, gt.ctypes.data))
+    k = nd.value
+    return dict(ijk=ijk[:k], dval=dval[:k], dg=dg[:k], vr=vr[:k] if itrans == 1 else None,
+                gt=gt[:k] if itrans == 1 else None, nd=k, nmerged=nm.value, nout=no.value)
 
 
 class Conditioning:

@@ -16,9 +16,11 @@
 ! library's writer streams to disk while later realizations are computed (ss3d_writer_*).
 !
 ! Options beyond the reference: --fp32 (FP32 realizations), --no-grid-cap (periodic grids
-! above the reference's 2^27-node limit).  Not part of this engine: conditioning (--cond,
-! icond=1), which stops with a message.  --vario computes the experimental variograms on the GPU,
-! chunk by chunk on the device copies of the realizations.
+! above the reference's 2^27-node limit).  --vario computes the experimental variograms on the
+! GPU, chunk by chunk on the device copies of the realizations.  Conditioning (--cond file, or
+! icond=1 with the block in the parameter file; f90:700-1004): the driver reads the block and the
+! GSLIB data file, the library prepares the data (ss3d_cond_prepare), computes the kriging weights
+! of every node once and conditions each chunk of realizations on the device before it is written.
 !=======================================================================
 program specsim3d
   use iso_c_binding
@@ -41,6 +43,18 @@ program specsim3d
   real(dp), allocatable :: zmean(:), zvar(:), zimag(:)
   integer :: r0, cnt, chunk
   integer(8) :: t0, t_spec, t_real, t_vario, t_end, rate, tv0, tv1, tvario
+  ! conditioning (block of f90:731-747 and the data file)
+  character(len=256) :: condfile
+  character(len=512) :: datafile, cline
+  type(ss3d_cond_columns_t) :: ccols
+  type(ss3d_cond_params_t) :: cpar
+  type(ss3d_cond_info_t) :: cinfo
+  type(c_ptr) :: cond
+  integer :: ciu, nvar, nraw, cnd
+  integer(c_int32_t) :: c_nd, c_nmerged, c_nout
+  real(dp), allocatable :: crows(:, :), crow(:)
+  real(dp), allocatable, target :: c_dval(:), c_dg(:), c_vr(:), c_gt(:)
+  integer(c_int32_t), allocatable :: c_ijk(:, :)
   ! output files (written by the library's streaming writer)
   type(c_ptr) :: wr, dfield
   type(ss3d_outputs_t), target :: outs
@@ -76,6 +90,7 @@ program specsim3d
     case ('--cond')
       ia = ia + 1
       if (ia > na) stop '--cond needs a conditioning parameter file'
+      call get_command_argument(ia, condfile)
       f_cond = .true.; ncli = ncli + 1
     case ('--fp32')
       f_fp32 = .true.
@@ -172,10 +187,8 @@ program specsim3d
       end if
     end if
   end if
+  if (f_cond) call cond_read()
   close(10)
-  if (f_cond) then
-    write(*,*) 'conditional simulation is not part of this engine (unconditional path only)'; stop 1
-  end if
 
   ! ---- periodic grid (library: sizing rule of the reference)
   call ss3d_check(ss3d_periodic_grid(st, int(nst, c_int), spc(1), spc(2), spc(3), int(nx, c_int), int(ny, c_int), &
@@ -220,6 +233,10 @@ program specsim3d
   if (f_vario) call vario_read(trim(varfile))
   call ss3d_check(ss3d_writer_open(ctx, outs, int(nx, c_int), int(ny, c_int), int(nz, c_int), int(nreal, c_int), &
                                    prec, wr), 'output files')
+  if (f_cond) then
+    call cond_setup()
+    call ss3d_check(ss3d_writer_condition(wr, cond), 'conditioning')
+  end if
 
   ! ---- realizations
   chunk = int(ss3d_writer_slot(wr))
@@ -241,6 +258,7 @@ program specsim3d
       '  max|Im|=', zimag(i)
   end do
   i = ss3d_spectrum_destroy(spec)
+  if (f_cond) call cond_report()
   call system_clock(t_real)
   write(*, '(a,f8.2,a)') ' time realizations         : ', real(t_real - t_spec - tvario)/real(rate), ' s'
   if (f_vario) then
@@ -251,12 +269,141 @@ program specsim3d
 
   ! ---- outputs: wait for the streamed files, write the GSLIB/ASCII files (same files as the reference)
   call ss3d_check(ss3d_writer_close(wr), 'output files')
+  if (f_cond) i = ss3d_cond_destroy(cond)
   i = ss3d_context_destroy(ctx)
 
   call system_clock(t_end)
   write(*, '(a,f8.2,a)') ' elapsed time: ', real(t_end - t0)/real(rate), ' s'
 
 contains
+
+  ! Reads the conditioning block (f90:731-749) from the parameter file (new format, unit 10, still
+  ! open) or from the --cond file (legacy), and the GSLIB data file (f90:757-779).
+  subroutine cond_read()
+    integer :: j, ir
+    if (legacy) then
+      ciu = 15
+      open(ciu, file=trim(condfile), status='old', action='read', iostat=ios)
+      if (ios /= 0) then
+        write(*,*) 'Conditioning parameter file not found: ', trim(condfile); stop 1
+      end if
+    else
+      ciu = 10
+    end if
+    read(ciu, '(a)') cline
+    cline = adjustl(cline)
+    datafile = cline(1:index(cline//' ', ' ') - 1)
+    read(ciu, *) ccols%ixc, ccols%iyc, ccols%izc, ccols%ivc, ccols%iwc
+    read(ciu, *) ccols%tmin, ccols%tmax
+    read(ciu, *) ccols%itrans, ccols%skmean
+    read(ciu, *) cpar%zlo, cpar%zhi
+    read(ciu, *) cpar%ltail, cpar%ltpar
+    read(ciu, *) cpar%utail, cpar%utpar
+    read(ciu, *) cpar%rmax, cpar%rmed, cpar%rvert
+    read(ciu, *) cpar%sa1, cpar%sa2, cpar%sa3
+    read(ciu, *) cpar%ndmax
+    if (legacy) close(ciu)
+    if (cpar%ndmax < 1 .or. cpar%ndmax > 100) stop 'cond: ndmax must be in 1..100'
+    if (cpar%rmax <= 0 .or. cpar%rmed <= 0 .or. cpar%rvert <= 0) stop 'cond: search radii must be positive'
+    cpar%itrans = ccols%itrans
+    cpar%skmean = ccols%skmean
+    open(16, file=trim(datafile), status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(*,*) 'Data file not found: ', trim(datafile); stop 1
+    end if
+    read(16, '(a)') cline
+    read(16, *) nvar
+    do j = 1, nvar
+      read(16, '(a)') cline
+    end do
+    nraw = 0
+    do
+      read(16, '(a)', iostat=ios) cline
+      if (ios /= 0) exit
+      if (len_trim(cline) > 0) nraw = nraw + 1
+    end do
+    rewind(16)
+    read(16, '(a)') cline
+    read(16, *) nvar
+    do j = 1, nvar
+      read(16, '(a)') cline
+    end do
+    allocate(crow(nvar), crows(nvar, max(nraw, 1)))
+    ir = 0
+    do j = 1, nraw
+      read(16, '(a)', iostat=ios) cline
+      if (ios /= 0) exit
+      if (len_trim(cline) == 0) cycle
+      read(cline, *) crow
+      ir = ir + 1
+      crows(:, ir) = crow
+    end do
+    nraw = ir
+    close(16)
+  end subroutine cond_read
+
+  ! Prepares the data (library, f90:760-846), writes <base>_nscore.trn and <base>_conddata.dat as
+  ! the reference does (f90:826-855) and computes the kriging weights of every node on the GPU.
+  subroutine cond_setup()
+    integer :: u, j, rc
+    allocate(c_ijk(3, max(nraw, 1)), c_dval(max(nraw, 1)), c_dg(max(nraw, 1)), c_vr(max(nraw, 1)), c_gt(max(nraw, 1)))
+    rc = ss3d_cond_prepare(crows, int(nraw, c_int), int(nvar, c_int), ccols, int(nx, c_int), int(ny, c_int), &
+                           int(nz, c_int), org(1), org(2), org(3), spc(1), spc(2), spc(3), c_nd, c_nmerged, c_nout, &
+                           c_ijk, c_dval, c_dg, c_vr, c_gt)
+    if (rc /= 0) then
+      if (ss3d_error_has('no conditioning data')) stop 'cond: no conditioning data inside the grid'
+    end if
+    call ss3d_check(rc, 'conditioning data')
+    cnd = int(c_nd)
+    if (ccols%itrans == 1) then
+      open(newunit=u, file=trim(outbase)//'_nscore.trn', status='replace', action='write')
+      write(u, '(a)') 'SPECSIM3D normal score transform table (value, normal score)'
+      write(u, '(i0)') 2
+      write(u, '(a)') 'value'
+      write(u, '(a)') 'nscore'
+      do j = 1, cnd
+        write(u, '(2es17.8)') c_vr(j), c_gt(j)
+      end do
+      close(u)
+      cpar%nt = c_nd
+      cpar%vr = c_loc(c_vr)
+      cpar%gt = c_loc(c_gt)
+    end if
+    open(newunit=u, file=trim(outbase)//'_conddata.dat', status='replace', action='write')
+    write(u, '(a)') 'SPECSIM3D conditioning data used (grid nodes)'
+    write(u, '(i0)') 5
+    write(u, '(a)') 'x'
+    write(u, '(a)') 'y'
+    write(u, '(a)') 'z'
+    write(u, '(a)') 'value'
+    write(u, '(a)') 'gauss'
+    do j = 1, cnd
+      write(u, '(5es17.8)') org(1) + (c_ijk(1, j) - 1)*spc(1), org(2) + (c_ijk(2, j) - 1)*spc(2), &
+                            org(3) + (c_ijk(3, j) - 1)*spc(3), c_dval(j), c_dg(j)
+    end do
+    close(u)
+    call ss3d_check(ss3d_cond_create(ctx, st, int(nst, c_int), nugget, int(nx, c_int), int(ny, c_int), &
+                                     int(nz, c_int), org(1), org(2), org(3), spc(1), spc(2), spc(3), c_nd, c_ijk, &
+                                     c_dg, cpar, cond, cinfo), 'kriging weights')
+  end subroutine cond_setup
+
+  ! The reference's conditioning lines (f90:818-819, 844-845, 890, 927-931).
+  subroutine cond_report()
+    character(len=24) :: tb
+    call ss3d_check(ss3d_cond_info_get(cond, cinfo), 'conditioning')
+    write(*, '(a,i8,a,i8,a,i8,a)') ' conditioning data: ', c_nd, ' nodes used (', c_nmerged, ' merged, ', c_nout, &
+      ' outside the grid)'
+    if (ccols%itrans == 1) write(*, '(a,f9.4,a,f9.4)') ' normal score transform: data mean=', &
+      sum(c_dg(1:cnd))/cnd, '  variance=', sum((c_dg(1:cnd) - sum(c_dg(1:cnd))/cnd)**2)/cnd
+    write(*, '(a,3i5,a,i0,a)') ' super blocks:', cinfo%nsbx, cinfo%nsby, cinfo%nsbz, '  (search ', cpar%ndmax, &
+      ' nearest data)'
+    write(*, '(a,f6.1,a,i0,a,i0,a)') ' mean neighbours per node: ', real(cinfo%nsum, dp)/(real(nx, dp)*ny*nz), &
+      '  (nodes without data: ', cinfo%nnone, ', data nodes: ', cinfo%nexact, ')'
+    if (cinfo%nfail > 0) write(*, '(a,i0)') ' WARNING: kriging systems regularized: ', cinfo%nfail
+    write(tb, '(f10.2)') real(cinfo%backtr_s)
+    write(*, '(a,f8.2,a)') ' time conditioning: ', real(cinfo%weights_s + cinfo%apply_s), &
+      ' s (kriging) + '//trim(adjustl(tb))//' s (back transform)'
+  end subroutine cond_report
 
   ! Reads the variogram file (f90:454-462): ndir iperreal / ndir lines sx sy sz nlag (lag k =
   ! k*(sx,sy,sz) cells).  A file that cannot be opened is reported where the reference reports it,

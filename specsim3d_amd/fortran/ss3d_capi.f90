@@ -9,11 +9,13 @@ module ss3d_capi
   use iso_c_binding
   implicit none
   private
-  public :: ss3d_structure_t, ss3d_spec_diag_t, ss3d_outputs_t, SS3D_FP64, SS3D_FP32
+  public :: ss3d_structure_t, ss3d_spec_diag_t, ss3d_outputs_t, ss3d_cond_columns_t, ss3d_cond_params_t, &
+            ss3d_cond_info_t, SS3D_FP64, SS3D_FP32
   public :: ss3d_context_create, ss3d_context_destroy, ss3d_periodic_grid, ss3d_spectrum_create, &
             ss3d_spectrum_destroy, ss3d_realize, ss3d_draws_per_realization, ss3d_check, &
             ss3d_variogram, ss3d_npairs, ss3d_covariance, ss3d_writer_open, ss3d_writer_slot, &
-            ss3d_writer_realize, ss3d_writer_close
+            ss3d_writer_realize, ss3d_writer_close, ss3d_cond_prepare, ss3d_cond_create, ss3d_cond_info_get, &
+            ss3d_cond_destroy, ss3d_writer_condition, ss3d_error_has
 
   integer(c_int), parameter :: SS3D_FP64 = 0, SS3D_FP32 = 1
 
@@ -34,6 +36,27 @@ module ss3d_capi
     integer(c_int32_t) :: vtk_count = 0, reserved = 0
     type(c_ptr) :: gslib_path = c_null_ptr, vtk_ascii_path = c_null_ptr
     real(c_double) :: origin(3) = 0, spacing(3) = 1
+  end type
+
+  ! conditioning (cond_mod, f90:548-1083): data columns, search and transform, what is printed
+  type, bind(C) :: ss3d_cond_columns_t
+    integer(c_int32_t) :: ixc = 1, iyc = 2, izc = 0, ivc = 3, iwc = 0, itrans = 0
+    real(c_double) :: tmin = -1.0d21, tmax = 1.0d21, skmean = 0
+  end type
+
+  type, bind(C) :: ss3d_cond_params_t
+    real(c_double) :: rmax = 1, rmed = 1, rvert = 1, sa1 = 0, sa2 = 0, sa3 = 0
+    integer(c_int32_t) :: ndmax = 16, itrans = 0
+    real(c_double) :: skmean = 0
+    integer(c_int32_t) :: nt = 0, ltail = 1, utail = 1, reserved = 0
+    type(c_ptr) :: vr = c_null_ptr, gt = c_null_ptr
+    real(c_double) :: zlo = 0, zhi = 0, ltpar = 1, utpar = 1
+  end type
+
+  type, bind(C) :: ss3d_cond_info_t
+    integer(c_int32_t) :: nsbx = 0, nsby = 0, nsbz = 0, nnone = 0, nexact = 0, nfail = 0
+    integer(c_int64_t) :: nsum = 0
+    real(c_double) :: weights_s = 0, apply_s = 0, backtr_s = 0
   end type
 
   interface
@@ -138,6 +161,49 @@ module ss3d_capi
       type(c_ptr), value :: w
     end function
 
+    integer(c_int) function ss3d_cond_prepare(rows, nrow, ncol, cols, nx, ny, nz, xmin, ymin, zmin, dx, dy, dz, &
+        nd, nmerged, nout, ijk, dval, dg, vr, gt) bind(C, name='ss3d_cond_prepare')
+      import :: c_int, c_int32_t, c_double, ss3d_cond_columns_t
+      real(c_double), intent(in) :: rows(*)
+      integer(c_int), value :: nrow, ncol, nx, ny, nz
+      type(ss3d_cond_columns_t), intent(in) :: cols
+      real(c_double), value :: xmin, ymin, zmin, dx, dy, dz
+      integer(c_int32_t), intent(out) :: nd, nmerged, nout
+      integer(c_int32_t), intent(out) :: ijk(*)
+      real(c_double), intent(out) :: dval(*), dg(*), vr(*), gt(*)
+    end function
+
+    integer(c_int) function ss3d_cond_create(ctx, st, nst, nugget, nx, ny, nz, xmin, ymin, zmin, dx, dy, dz, nd, &
+        ijk, g, p, cond, info) bind(C, name='ss3d_cond_create')
+      import :: c_int, c_int32_t, c_double, c_ptr, ss3d_structure_t, ss3d_cond_params_t, ss3d_cond_info_t
+      type(c_ptr), value :: ctx
+      type(ss3d_structure_t), intent(in) :: st(*)
+      integer(c_int), value :: nst, nx, ny, nz
+      integer(c_int32_t), value :: nd
+      real(c_double), value :: nugget, xmin, ymin, zmin, dx, dy, dz
+      integer(c_int32_t), intent(in) :: ijk(*)
+      real(c_double), intent(in) :: g(*)
+      type(ss3d_cond_params_t), intent(in) :: p
+      type(c_ptr), intent(out) :: cond
+      type(ss3d_cond_info_t), intent(out) :: info
+    end function
+
+    integer(c_int) function ss3d_cond_info_get(cond, info) bind(C, name='ss3d_cond_info_get')
+      import :: c_int, c_ptr, ss3d_cond_info_t
+      type(c_ptr), value :: cond
+      type(ss3d_cond_info_t), intent(out) :: info
+    end function
+
+    integer(c_int) function ss3d_cond_destroy(cond) bind(C, name='ss3d_cond_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: cond
+    end function
+
+    integer(c_int) function ss3d_writer_condition(w, cond) bind(C, name='ss3d_writer_condition')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: w, cond
+    end function
+
     integer(c_int) function ss3d_last_error_c(buf, length) bind(C, name='ss3d_last_error')
       import :: c_int, c_int64_t, c_char
       character(kind=c_char), intent(out) :: buf(*)
@@ -146,6 +212,22 @@ module ss3d_capi
   end interface
 
 contains
+
+  ! True when the library's last error message on this thread contains s.
+  logical function ss3d_error_has(s)
+    character(len=*), intent(in) :: s
+    character(kind=c_char) :: buf(512)
+    character(len=512) :: msg
+    integer :: i, rc
+    buf = c_null_char
+    rc = ss3d_last_error_c(buf, int(size(buf), c_int64_t))
+    msg = ''
+    do i = 1, size(buf)
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+    ss3d_error_has = index(msg, s) > 0
+  end function ss3d_error_has
 
   ! Stops the program with the library's message when rc is not 0.
   subroutine ss3d_check(rc, what)

@@ -53,7 +53,32 @@ def test_grid_cap_kept(exe, tmp_path):
     assert "periodic grid too large (>2^27 nodes)" in r.stdout + r.stderr
 
 
-def test_conditioning_is_refused(exe, tmp_path):
-    (tmp_path / "p.par").write_text(par_text(**GOOD, base="x", codes=[3], icond=1))
-    r = run(exe, str(tmp_path), "p.par")
-    assert r.returncode == 1 and "conditional simulation is not part of this engine" in r.stdout
+COND_BLOCK = "d.dat\n1 2 3 4 0\n-1e21 1e21\n0 0.0\n-5 5\n1 0\n1 0\n{r}\n0 0 0\n{nd}\n"
+COND_CASES = {  # conditioning block errors (f90:734-761): message, exit code, how the block is given
+    "ndmax": (COND_BLOCK.format(r="10 10 10", nd=0), "cond: ndmax must be in 1..100", 0, "file"),
+    "radii": (COND_BLOCK.format(r="10 0 10", nd=8), "cond: search radii must be positive", 0, "file"),
+    "datafile": (COND_BLOCK.format(r="10 10 10", nd=8), "Data file not found: d.dat", 1, "inline"),
+    "condfile": (None, "Conditioning parameter file not found: missing.cpar", 1, "cli"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(COND_CASES))
+def test_conditioning_block_errors(exe, case, tmp_path):
+    """The driver reads the conditioning block before the simulation, the reference after it: the
+    error line and the exit code are the same, the reference has printed its realizations first."""
+    block, msg, rc, how = COND_CASES[case]
+    if how == "inline":
+        (tmp_path / "p.par").write_text(par_text(**GOOD, base="x", codes=[3], icond=1) + block)
+        args = []
+    else:
+        (tmp_path / "p.par").write_text(par_text(**GOOD, base="x"))
+        if block is not None:
+            (tmp_path / "c.cpar").write_text(block)
+        args = ["--bin", "--cond", "c.cpar" if block is not None else "missing.cpar"]
+    r = run(exe, str(tmp_path), "p.par", *args)
+    out = r.stdout + r.stderr
+    assert r.returncode == rc and msg in out, (r.returncode, out)
+    assert "realization" not in out  # stopped before the engine was opened
+    if os.path.exists(REF_EXE):
+        q = run(REF_EXE, str(tmp_path), "p.par", *args)
+        assert q.returncode == r.returncode and msg in q.stdout + q.stderr

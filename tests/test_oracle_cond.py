@@ -181,3 +181,20 @@ def test_ridge_rule(case):
             # the 1e3-fold ridge leaves the system nearly singular (weights ~1e5): relative check
             assert np.abs(res[r].ravel()[mask] - want[mask]).max() < 1e-7 * np.abs(want[mask]).max()
     assert np.abs(res.reshape(2, -1)[:, at] - v[None]).max() < 1e-15
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_library_data_preparation_equals_oracle(name):
+    """ss3d_cond_prepare (the product's host code) gives the oracle's used data bit for bit."""
+    import specsim3d_amd as S
+
+    c, unc, res, out = run_case(name)
+    p = c["params"]
+    got = S.cond_prepare(c["rows"], c["grid"], c["origin"], c["spacing"], p["ixc"], p["iyc"], p["izc"], p["ivc"],
+                         p["iwc"], p["tmin"], p["tmax"], p["itrans"], p["skmean"])
+    info = out["info"]
+    assert (got["nd"], got["nmerged"], got["nout"]) == (info["nd"], info["nmerged"], info["nout"])
+    assert np.array_equal(got["ijk"], out["ijk"])
+    assert np.array_equal(got["dval"], out["dval"]) and np.array_equal(got["dg"], out["dg"])
+    if p["itrans"] == 1:
+        assert np.array_equal(got["vr"], out["vrt"]) and np.array_equal(got["gt"], out["gtt"])
