@@ -335,6 +335,82 @@ def writer_leg(S, eng, spec, nreal):
     return out
 
 
+def conditioning_leg(S, eng, hip, spec, nreal, nd=2000, ndmax=16):
+    """Conditioning of the workload's realizations on nd data at random nodes (values N(0, 1), simple
+    kriging, ndmax nearest data in the model's anisotropic ellipsoid): the kriging weights of all
+    256^3 nodes once, then each step's 64 FP32 realizations conditioned on the device.  The CPU side
+    is the reference program's own conditioning (its 'time conditioning' kriging timer) on a bounded
+    sample: 128x128x64 with the same data density and search, 2 realizations."""
+    import numpy as np
+
+    nx, ny, nz = GRID
+    ncell = nx * ny * nz
+    rng = np.random.default_rng(11)
+    idx = rng.choice(ncell, nd, replace=False)
+    i, j, k = np.unravel_index(idx, (nz, ny, nx))[::-1]
+    rows = np.c_[i, j, k, rng.normal(size=nd)].astype(float)
+    s0 = STRUCTS[0]
+    prep = S.cond_prepare(rows, GRID, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1, 2, 3, 4)
+    t0 = time.perf_counter()
+    cond = eng.conditioning(STRUCTS, NUGGET, GRID, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), prep["ijk"], prep["dg"],
+                            s0[2], s0[3], s0[4], (s0[5], s0[6], s0[7]), ndmax)
+    t_weights = time.perf_counter() - t0
+    info = cond.info()
+    out = hip.malloc(nreal * ncell * 4)
+    try:
+        eng.realize_device(spec, ISEED, IGAUSS, 0, nreal, nx, ny, nz, S.FP32, out)
+        cond.apply(out, S.FP32, nreal)  # warm-up
+        eng.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for r in range(reps):
+            eng.realize_device(spec, ISEED, IGAUSS, r * nreal, nreal, nx, ny, nz, S.FP32, out)
+            cond.apply(out, S.FP32, nreal)
+        eng.synchronize()
+        t_step = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for r in range(reps):
+            cond.apply(out, S.FP32, nreal)
+        eng.synchronize()
+        t_apply = (time.perf_counter() - t0) / reps
+    finally:
+        hip.free(out)
+        cond.close()
+    res = {"workload": f"{nx}x{ny}x{nz}, {nd} data at random nodes, ndmax {ndmax}, search {s0[2]}/{s0[3]}/{s0[4]} "
+                       f"az {s0[5]} dip {s0[6]}, simple kriging, {nreal} FP32 realizations per step",
+           "weights_s": round(t_weights, 3), "weights_Mnodes_s": round(ncell / t_weights / 1e6, 1),
+           "mean_neighbours": round(info["nsum"] / ncell, 2), "apply_ms_per_step": round(t_apply * 1e3, 2),
+           "step_with_conditioning_ms": round(t_step * 1e3, 2),
+           "gcells_s_conditioned": round(nreal * ncell / t_step / 1e9, 2)}
+    exe = os.path.join(ROOT, "oracle", "_ref", "specsim3d")
+    if os.path.exists(exe):
+        g = (128, 128, 64)
+        nds = max(1, int(round(nd * (g[0] * g[1] * g[2]) / ncell)))
+        idx = rng.choice(g[0] * g[1] * g[2], nds, replace=False)
+        ci, cj, ck = np.unravel_index(idx, (g[2], g[1], g[0]))[::-1]
+        with tempfile.TemporaryDirectory() as wd:
+            with open(os.path.join(wd, "d.dat"), "w") as fh:
+                fh.write("d\n4\nx\ny\nz\nv\n" + "".join("%d %d %d %.8g\n" % (a, b, c, v) for a, b, c, v in
+                                                             zip(ci, cj, ck, rng.normal(size=nds))))
+            with open(os.path.join(wd, "c.par"), "w") as fh:
+                fh.write("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d 2 1 %d\nc\n1\n3\n1\n"
+                         % (*g, NUGGET, s0[0], *s0[1:], ISEED, IGAUSS) +
+                         "d.dat\n1 2 3 4 0\n-1e21 1e21\n0 0.0\n-5 5\n1 0\n1 0\n%r %r %r\n%r %r %r\n%d\n"
+                         % (s0[2], s0[3], s0[4], s0[5], s0[6], s0[7], ndmax))
+            thr = min(os.cpu_count() or 1, 16)
+            r = subprocess.run([exe, "c.par"], cwd=wd, capture_output=True, text=True,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(thr)), timeout=900)
+        m = re.search(r"time conditioning:\s+([\d.]+) s \(kriging\)", r.stdout) if r.returncode == 0 else None
+        if m:
+            tk = float(m.group(1))
+            nodes = g[0] * g[1] * g[2]
+            res["cpu_reference"] = {"sample": f"{g[0]}x{g[1]}x{g[2]}, {nds} data, ndmax {ndmax}, 2 realizations",
+                                    "threads": thr, "kriging_s": tk,
+                                    "Mnodes_s": round(nodes / max(tk, 1e-9) / 1e6, 3)}
+            res["weights_speedup_vs_reference"] = round(res["weights_Mnodes_s"] / res["cpu_reference"]["Mnodes_s"], 1)
+    return res
+
+
 def spectrum_file_leg(S, eng, spec, grid, build_ms):
     """The workload's spectral density saved to and loaded from a file (ss3d_spectrum_save/load), timed
     against building it on the device."""
@@ -382,6 +458,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-writer", action="store_true")
+    ap.add_argument("--no-cond", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
@@ -450,6 +527,7 @@ def main():
     hip.free(stats)
     writer = None if args.no_writer else writer_leg(S, eng, spec, nreal)
     specfile = None if args.no_writer else spectrum_file_leg(S, eng, spec, grid, spec_ms)
+    conditioning = None if args.no_cond else conditioning_leg(S, eng, hip, spec, nreal)
     spec.close()
 
     uneven = None if args.no_batch else uneven_batch(S, eng, 3)
@@ -497,6 +575,7 @@ def main():
         "large_job": large,
         "output_files": writer,
         "spectrum_file": specfile,
+        "conditioning": conditioning,
         "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
                         "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

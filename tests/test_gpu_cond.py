@@ -217,3 +217,26 @@ def test_writer_conditions_chunks(tmp_path, eng):
     assert np.abs(cf - want).max() / np.abs(want).max() < 1e-10
     k.close()
     sp.close()
+
+
+@pytest.mark.parametrize("nreal", [19, 40])
+def test_many_realizations_per_call(eng, nreal):
+    """Calls with 16..31 and 32+ realizations sum 16 or 32 realizations per pass over the weights
+    (plus a remainder): the same values as the oracle's."""
+    say(f"[cond] {nreal} realizations per call")
+    c = G.cond_case("ns2d")
+    npg = S.periodic_grid(c["sts"], *c["spacing"], *c["grid"], c["ipad"])
+    sp = eng.spectrum(c["sts"], c["nugget"], *c["spacing"], *npg)
+    f = eng.realize(sp, 99, 1, 0, nreal, *c["grid"], precision=S.FP64)[0]
+    sp.close()
+    want, out = prepared(c, f)
+    k = device_cond(eng, c, out)
+    d = Dev(f)
+    try:
+        k.apply(d.p.value, S.FP64, nreal)
+        eng.synchronize()
+        got = d.get()
+    finally:
+        d.free()
+    k.close()
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-10
