@@ -26,7 +26,10 @@ constexpr int nch(int L) { return L == 0 ? 0 : (L + el(L) - 1) / el(L); }  // ch
 constexpr int ch_s(int L, int j) { return el(L) * j; }                     // stages done before chunk j
 constexpr int ch_rc(int L, int j) { return L - el(L) * j < el(L) ? L - el(L) * j : el(L); }
 constexpr int strided_P(int L) { return L <= 6 ? 64 : (4096 >> L); }     // columns per strided tile
-constexpr int rows_P(int L) { return L <= 3 ? 64 : (4096 >> L); }        // rows per contiguous tile
+// rows per contiguous tile: 4 rows of 512 in a 256-thread block; the row-pair kernel holds two
+// rows and their draws in registers (158 VGPRs, 3 waves per SIMD), so smaller blocks let three
+// of them share a CU (2.03 -> 1.58 ms per 512^2-row launch of 2 realizations)
+constexpr int rows_P(int L) { return L <= 3 ? 64 : (L >= 12 ? 1 : (2048 >> L)); }
 
 // twiddle table W_n^m, m < n/2, from HBM or from its LDS copy
 struct TwG {
