@@ -57,6 +57,9 @@ typedef struct ss3d_spec_diag {
 typedef struct ss3d_context ss3d_context;
 typedef struct ss3d_spectrum ss3d_spectrum;
 
+/* ABI version, major * 10000 + minor * 100: 1.1 adds the output writer, spectrum files and
+ * conditioning to 1.0 (grid sizing, spectra, realizations, variograms, request loop, batch graph). */
+#define SS3D_VERSION 10100
 int ss3d_version(void);
 
 /* Copies the last error message of the calling thread (NUL-terminated, truncated to len). */

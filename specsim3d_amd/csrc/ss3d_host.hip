@@ -525,7 +525,7 @@ int check_grid(int npx, int npy, int npz) {
 // ================================================================ C ABI
 extern "C" {
 
-int ss3d_version(void) { return 10000; }
+int ss3d_version(void) { return SS3D_VERSION; }
 
 int ss3d_last_error(char* buf, int64_t len) {
     if (!buf || len <= 0) return SS3D_EINVAL;
