@@ -73,8 +73,59 @@ def par_text(nreal, base):
             % (*GRID, NUGGET, s[0], *s[1:], ISEED, nreal, IGAUSS, base))
 
 
+def reference_writer(exe, threads):
+    """The reference program's GSLIB writer: 8 FP64 realizations of 128x128x64 to .out (its elapsed
+    time minus its spectrum and realization timers)."""
+    g, nr = (128, 128, 64), 8
+    s0 = STRUCTS[0]
+    with tempfile.TemporaryDirectory() as wd:
+        with open(os.path.join(wd, "g.par"), "w") as fh:
+            fh.write("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d %d 1 %d\ng\n1\n1\n0\n"
+                     % (*g, NUGGET, s0[0], *s0[1:], ISEED, nr, IGAUSS))
+        r = subprocess.run([exe, "g.par"], cwd=wd, capture_output=True, text=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=600)
+    if r.returncode != 0:
+        return None
+    num = lambda pat: float(re.search(pat, r.stdout).group(1))
+    tw = num(r"elapsed time:\s+([\d.]+)") - num(r"realizations\s+:\s+([\d.]+)") - \
+        num(r"covariance\+spectrum:\s+([\d.]+)")
+    return {"sample": "GSLIB .out of 8 FP64 realizations of 128x128x64", "seconds": round(tw, 2)}
+
+
+def reference_conditioning(exe, threads, nd=2000, ndmax=16):
+    """The reference program's own kriging ('time conditioning', kriging part) on 128x128x64 with the
+    data density, search and ndmax of the GPU conditioning leg, 2 realizations."""
+    import numpy as np
+
+    g = (128, 128, 64)
+    nodes = g[0] * g[1] * g[2]
+    nds = max(1, int(round(nd * nodes / (GRID[0] * GRID[1] * GRID[2]))))
+    rng = np.random.default_rng(12)
+    idx = rng.choice(nodes, nds, replace=False)
+    ci, cj, ck = np.unravel_index(idx, (g[2], g[1], g[0]))[::-1]
+    s0 = STRUCTS[0]
+    with tempfile.TemporaryDirectory() as wd:
+        with open(os.path.join(wd, "d.dat"), "w") as fh:
+            fh.write("d\n4\nx\ny\nz\nv\n" + "".join("%d %d %d %.8g\n" % (a, b, c, v) for a, b, c, v in
+                                                         zip(ci, cj, ck, rng.normal(size=nds))))
+        with open(os.path.join(wd, "c.par"), "w") as fh:
+            fh.write("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d 2 1 %d\nc\n1\n3\n1\n"
+                     % (*g, NUGGET, s0[0], *s0[1:], ISEED, IGAUSS) +
+                     "d.dat\n1 2 3 4 0\n-1e21 1e21\n0 0.0\n-5 5\n1 0\n1 0\n%r %r %r\n%r %r %r\n%d\n"
+                     % (s0[2], s0[3], s0[4], s0[5], s0[6], s0[7], ndmax))
+        r = subprocess.run([exe, "c.par"], cwd=wd, capture_output=True, text=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=900)
+    m = re.search(r"time conditioning:\s+([\d.]+) s \(kriging\)", r.stdout) if r.returncode == 0 else None
+    if not m:
+        return None
+    tk = float(m.group(1))
+    return {"sample": f"{g[0]}x{g[1]}x{g[2]}, {nds} data, ndmax {ndmax}, 2 realizations", "kriging_s": tk,
+            "Mnodes_s": round(nodes / max(tk, 1e-9) / 1e6, 3)}
+
+
 def cpu_baseline(threads):
-    """Reference program on the host cores: 256^3 (512^3 periodic), one realization."""
+    """Reference program on the host cores: 256^3 (512^3 periodic), one realization; plus its GSLIB
+    writer and its conditioning on bounded samples (the CPU sides of the output and conditioning legs)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "specsim3d")
     ncell = GRID[0] * GRID[1] * GRID[2]
     if os.path.exists(exe):
@@ -90,7 +141,9 @@ def cpu_baseline(threads):
             ts = float(re.search(r"covariance\+spectrum:\s+([\d.]+)", r.stdout).group(1))
             return {"value": ncell / tr / 1e9, "unit": "Gcells/s", "cores": threads, "kind": "reference",
                     "sample": f"reference program (amdflang -O3 -fopenmp), 256^3 grid / 512^3 periodic, 1 realization: "
-                              f"time realizations {tr:.2f} s, covariance+spectrum {ts:.2f} s, wall {wall:.1f} s"}
+                              f"time realizations {tr:.2f} s, covariance+spectrum {ts:.2f} s, wall {wall:.1f} s",
+                    "gslib_writer": reference_writer(exe, threads),
+                    "conditioning": reference_conditioning(exe, threads)}
     # fallback: the oracle's C restatement, one thread, 128^3 grid (256^3 periodic)
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
@@ -262,8 +315,8 @@ def writer_leg(S, eng, spec, nreal):
     FP64 fields: streamed by the library's writer (device compute, device-to-host copy and file
     writes overlapped) vs the serial order of the reference (every realization to host memory,
     then the file).  (2) GSLIB text (.out, es15.7, every realization of a node per line) of 8
-    FP64 realizations of 128x128x64 through the writer, and the reference program's own run of the
-    same case (its elapsed time minus its spectrum and realization timers is its writer)."""
+    FP64 realizations of 128x128x64 through the writer (the reference program's own writer on the
+    same case is timed by the CPU baseline)."""
     import numpy as np
 
     nx, ny, nz = GRID
@@ -311,22 +364,6 @@ def writer_leg(S, eng, spec, nreal):
         sp.close()
         text = {"case": "128x128x64, 8 FP64 realizations, es15.7", "seconds": round(t_text, 3),
                 "MB_s": round(size / t_text / 1e6, 1), "file_MB": round(size / 1e6, 1)}
-        exe = os.path.join(ROOT, "oracle", "_ref", "specsim3d")
-        if os.path.exists(exe):
-            s0 = STRUCTS[0]
-            with open(os.path.join(wd, "g.par"), "w") as fh:
-                fh.write("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d %d 1 %d\ng\n1\n1\n0\n"
-                         % (*g, NUGGET, s0[0], *s0[1:], ISEED, nr, IGAUSS))
-            thr = min(os.cpu_count() or 1, 16)
-            r = subprocess.run([exe, "g.par"], cwd=wd, capture_output=True, text=True,
-                               env=dict(os.environ, OMP_NUM_THREADS=str(thr)), timeout=600)
-            if r.returncode == 0:
-                num = lambda pat: float(re.search(pat, r.stdout).group(1))
-                tw = num(r"elapsed time:\s+([\d.]+)") - num(r"realizations\s+:\s+([\d.]+)") - \
-                    num(r"covariance\+spectrum:\s+([\d.]+)")
-                text["reference_writer_s"] = round(tw, 2)
-                text["reference_threads"] = thr
-                text["speedup_vs_reference_writer"] = round(tw / t_text, 1)
         out["gslib_text"] = text
     finally:
         import shutil
@@ -338,9 +375,8 @@ def writer_leg(S, eng, spec, nreal):
 def conditioning_leg(S, eng, hip, spec, nreal, nd=2000, ndmax=16):
     """Conditioning of the workload's realizations on nd data at random nodes (values N(0, 1), simple
     kriging, ndmax nearest data in the model's anisotropic ellipsoid): the kriging weights of all
-    256^3 nodes once, then each step's 64 FP32 realizations conditioned on the device.  The CPU side
-    is the reference program's own conditioning (its 'time conditioning' kriging timer) on a bounded
-    sample: 128x128x64 with the same data density and search, 2 realizations."""
+    256^3 nodes once, then each step's 64 FP32 realizations conditioned on the device (the reference
+    program's own kriging on a bounded sample is timed by the CPU baseline)."""
     import numpy as np
 
     nx, ny, nz = GRID
@@ -382,32 +418,6 @@ def conditioning_leg(S, eng, hip, spec, nreal, nd=2000, ndmax=16):
            "mean_neighbours": round(info["nsum"] / ncell, 2), "apply_ms_per_step": round(t_apply * 1e3, 2),
            "step_with_conditioning_ms": round(t_step * 1e3, 2),
            "gcells_s_conditioned": round(nreal * ncell / t_step / 1e9, 2)}
-    exe = os.path.join(ROOT, "oracle", "_ref", "specsim3d")
-    if os.path.exists(exe):
-        g = (128, 128, 64)
-        nds = max(1, int(round(nd * (g[0] * g[1] * g[2]) / ncell)))
-        idx = rng.choice(g[0] * g[1] * g[2], nds, replace=False)
-        ci, cj, ck = np.unravel_index(idx, (g[2], g[1], g[0]))[::-1]
-        with tempfile.TemporaryDirectory() as wd:
-            with open(os.path.join(wd, "d.dat"), "w") as fh:
-                fh.write("d\n4\nx\ny\nz\nv\n" + "".join("%d %d %d %.8g\n" % (a, b, c, v) for a, b, c, v in
-                                                             zip(ci, cj, ck, rng.normal(size=nds))))
-            with open(os.path.join(wd, "c.par"), "w") as fh:
-                fh.write("0 0 0\n1 1 1\n%d %d %d\n%r 1\n%d %r %r %r %r %r %r %r\n%d 2 1 %d\nc\n1\n3\n1\n"
-                         % (*g, NUGGET, s0[0], *s0[1:], ISEED, IGAUSS) +
-                         "d.dat\n1 2 3 4 0\n-1e21 1e21\n0 0.0\n-5 5\n1 0\n1 0\n%r %r %r\n%r %r %r\n%d\n"
-                         % (s0[2], s0[3], s0[4], s0[5], s0[6], s0[7], ndmax))
-            thr = min(os.cpu_count() or 1, 16)
-            r = subprocess.run([exe, "c.par"], cwd=wd, capture_output=True, text=True,
-                               env=dict(os.environ, OMP_NUM_THREADS=str(thr)), timeout=900)
-        m = re.search(r"time conditioning:\s+([\d.]+) s \(kriging\)", r.stdout) if r.returncode == 0 else None
-        if m:
-            tk = float(m.group(1))
-            nodes = g[0] * g[1] * g[2]
-            res["cpu_reference"] = {"sample": f"{g[0]}x{g[1]}x{g[2]}, {nds} data, ndmax {ndmax}, 2 realizations",
-                                    "threads": thr, "kriging_s": tk,
-                                    "Mnodes_s": round(nodes / max(tk, 1e-9) / 1e6, 3)}
-            res["weights_speedup_vs_reference"] = round(res["weights_Mnodes_s"] / res["cpu_reference"]["Mnodes_s"], 1)
     return res
 
 
@@ -584,7 +594,14 @@ def main():
         "passes": passes,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(min(os.cpu_count() or 1, 16))
+        cb = cpu_baseline(min(os.cpu_count() or 1, 16))
+        line["cpu_baseline"] = cb
+        if writer and cb.get("gslib_writer"):
+            t = writer["gslib_text"]
+            t["speedup_vs_reference_writer"] = round(cb["gslib_writer"]["seconds"] / t["seconds"], 1)
+        if conditioning and cb.get("conditioning"):
+            conditioning["weights_speedup_vs_reference"] = round(
+                conditioning["weights_Mnodes_s"] / cb["conditioning"]["Mnodes_s"], 1)
     print(json.dumps(line), flush=True)
 
 
