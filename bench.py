@@ -156,7 +156,7 @@ def cpu_baseline(threads):
 
 
 def serve(S, eng, rate, duration, seed, nreal=64, nslots=4, spot_frac=0.05, repeat_frac=0.3):
-    """Poisson arrivals of config-2-like requests (256^3, igauss=1, FP32 output left in HBM): 30% repeat
+    """Poisson arrivals of 256^3 requests like the headline workload (igauss=1, FP32 output left in HBM): 30% repeat
     an earlier model (spectral-density cache), the rest draw ranges and angles anew; 5% carry an FP64 spot
     check of realization 0.  Returns latency percentiles and throughput."""
     import threading
@@ -474,7 +474,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "Realizations are independent; see DESIGN.md for how the path would shard.", file=sys.stderr)
+              "Realizations are independent: a multi-GPU run would give each GPU a range of them.", file=sys.stderr)
         sys.exit(2)
 
     import specsim3d_amd as S

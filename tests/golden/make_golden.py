@@ -32,7 +32,7 @@ EXE64 = os.path.join(ROOT, "oracle", "_ref", "specsim3d_f64")
 
 # name: (origin, spacing, grid, nugget, structures, iseed, nreal, ipad, igauss)
 CASES = {
-    # config 1: 64x64x16 spherical 20/10/5 az 45, ipad=1, igauss=0, seed 69069, one realization
+    # 64x64x16 spherical 20/10/5 az 45, ipad=1, igauss=0, seed 69069, one realization
     "c1_sph_pad": ((0, 0, 0), (1, 1, 1), (64, 64, 16), 0.0, [(1, 1.0, 20, 10, 5, 45, 0, 0)], 69069, 1, 1, 0),
     "exp_nug_gauss": ((0, 0, 0), (1, 1, 1), (24, 20, 12), 0.3, [(2, 0.7, 8, 4, 2, 30, 15, 0)], 4711, 3, 1, 1),
     "gau_rot": ((0.5, 0.5, 0.5), (1, 1, 1), (16, 16, 8), 0.0, [(3, 1.0, 6, 3, 2, 120, -20, 10)], 12345, 2, 1, 0),

@@ -115,7 +115,7 @@ def test_realization_offsets_are_independent(eng, precision, igauss):
     assert np.array_equal(full[1:4], mid)
 
 
-def test_config2_full_size_fp32_vs_fp64(eng):
+def test_256cubed_full_size_fp32_vs_fp64(eng):
     """256^3 in a 512^3 periodic grid (exponential 0.7 + nugget 0.3, ranges 80/40/10, az 30, dip 15,
     igauss=1): FP32 vs FP64 from the same draws, sill reproduced by the sample variance."""
     sts = [(2, 0.7, 80, 40, 10, 30, 15, 0)]
