@@ -167,6 +167,38 @@ __global__ __launch_bounds__((1 << L) / (1 << el(L)) * strided_P(L)) void stride
                     }
                 }
         }
+    } else if constexpr (OUT == OUT_SPEC) {
+        // spectral density of the last forward pass: S = c/N, amp = sqrt(S) where Re S >= 0, else 0,
+        // per-block max|Im S|, max|Re S|, sum of negative and of positive S (f90:1272-1288), the
+        // operations of the generic kernel's OUT_SPEC stage
+        const double rn = (double)a.npx * (double)a.npy * (double)a.npz;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < NBl; ++i)
+#pragma unroll
+                for (int bb = 0; bb < (1 << RCl); ++bb) {
+                    const int pos = g + G * i + (bb << Sl);
+                    const c64 c = v[i * (1 << RCl) + bb];
+                    const int j = a.axis == 1 ? pos : third, k = a.axis == 1 ? third : pos;
+                    const double sr = c.x / rn, si = c.y / rn;
+                    acc[0] = fmax(acc[0], fabs(si));
+                    acc[1] = fmax(acc[1], fabs(sr));
+                    double amp;
+                    if (sr < 0.0) {
+                        acc[2] += sr;
+                        amp = 0.0;
+                    } else {
+                        acc[3] += sr;
+                        amp = sqrt(sr);
+                    }
+                    const int64_t row = j + (int64_t)a.npy * k;
+                    a.amp64_out[i0 + (int64_t)a.npx * row] = amp;
+                    if (a.amp32h_out && i0 < a.hx) a.amp32h_out[i0 + (int64_t)a.hxa * row] = (float)(amp * kAmp32Scale);
+                }
+        }
+        const int ops[4] = {1, 1, 0, 0};
+        block_reduce_write<NT, 4>(acc, ops, a.partial + 4 * ((int64_t)b * a.nblocks + blockIdx.x));
     } else {  // OUT_REAL
         double acc[3] = {0.0, 0.0, 0.0};
         if (active) {
@@ -354,6 +386,60 @@ __global__ __launch_bounds__((1 << L) / (1 << el(L)) * rows_P(L)) void rows_pair
     if (pairs) {  // block-uniform
         __syncthreads();  // LDS reads of row R done
         finish(w, rowp);
+    }
+}
+
+// ------------------------------------------------------------ x pass of the spectral density
+// Rows of the covariance grid generated in registers at the minimum-image lags (f90:1256-1268,
+// lag n/2 stays +n/2), as the generic kernel's IN_COV stage does, then transformed with the
+// table in a.tw (forward) and stored complete.
+template <int L>
+__global__ __launch_bounds__((1 << L) / (1 << el(L)) * rows_P(L)) void rows_cov_kernel(PassArgs a) {
+    constexpr int N = 1 << L, EPT = 1 << el(L), G = N / EPT, P = rows_P(L), NT = G * P, M = nch(L);
+    constexpr int ROW = N + (N >> 3);
+    c64* lds = reinterpret_cast<c64*>(ss3d_smem);
+    c64* twl = lds + (size_t)ROW * P;
+    const int tid = threadIdx.x;
+    const int g = tid % G, q = tid / G;
+    const int row = xcd_remap(blockIdx.x, gridDim.x) * P + q;
+    const bool active = row < a.nrows;
+    const c64* __restrict__ tw = reinterpret_cast<const c64*>(a.tw);
+    if constexpr (M > 0) {
+        for (int m = tid; m < N / 2; m += NT) twl[m] = tw[m];
+        __syncthreads();
+    }
+    c64 v[EPT];
+    if (active) {
+        const int j = row % a.npy, k = row / a.npy;
+        const int jp = j > a.npy / 2 ? j - a.npy : j;
+        const int kp = k > a.npz / 2 ? k - a.npz : k;
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) {
+            const int pos = g + r * G;
+            const int ip = pos > a.npx / 2 ? pos - a.npx : pos;
+            v[r] = c64{cov64(a, (double)ip * a.dx, (double)jp * a.dy, (double)kp * a.dz), 0.0};
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) v[r] = c64{0.0, 0.0};
+    }
+    c64* lrow = lds + (size_t)q * ROW;
+    transform<L, G>(v, g, lrow, [](int pos) { return pos + (pos >> 3); }, TwL{twl});
+    if (active) {
+        constexpr int RCl = M ? ch_rc(L, M - 1) : 0, Sl = M ? ch_s(L, M - 1) : 0, NBl = EPT >> RCl;
+        c64* __restrict__ dst = reinterpret_cast<c64*>(a.out) + (int64_t)row * a.out_row;
+        const int nout = a.n_out;
+#pragma unroll
+        for (int i = 0; i < NBl; ++i)
+#pragma unroll
+            for (int bb = 0; bb < (1 << RCl); ++bb) {
+                const int pos = g + G * i + (bb << Sl);
+                if (pos < nout) {
+                    using d2 = double __attribute__((ext_vector_type(2)));
+                    const c64 z = v[i * (1 << RCl) + bb];
+                    __builtin_nontemporal_store((d2){z.x, z.y}, reinterpret_cast<d2*>(dst + pos));
+                }
+            }
     }
 }
 

@@ -113,7 +113,13 @@ int check_structures(const ss3d_structure* st, int nst) {
 }
 
 // ---------------------------------------------------------------- Park-Miller on the host
-uint32_t pm_mul(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) % kPmM); }
+// a b mod (2^31 - 1) for a, b < 2^31 - 1: since 2^31 = 1 (mod M), p = hi 2^31 + lo = hi + lo, one
+// conditional subtraction (no division)
+uint32_t pm_mul(uint32_t a, uint32_t b) {
+    const uint64_t p = (uint64_t)a * b;
+    const uint32_t s = (uint32_t)(p & 0x7fffffffu) + (uint32_t)(p >> 31);
+    return s >= kPmM ? s - kPmM : s;
+}
 uint32_t pm_pow(uint32_t a, uint64_t e) {
     e %= (kPmM - 1);  // a^(M-1) = 1, M prime
     uint32_t r = 1, x = a % kPmM;
@@ -397,7 +403,8 @@ struct Pass {
     PassArgs a;
     int kind;      // profiling kind (see ss3d_context_profile)
     int fast = 0;  // FP32 fast kernels: 1 strided, 2 C2R x pass, 3 one-point x axis;
-                   // FP64 compile-time kernels: 4 x pass (fill + rows), 5 strided
+                   // FP64 compile-time kernels: 4 x pass (fill + rows), 5 strided,
+                   // 6 x pass of the spectral density (covariance rows)
     int rm = 5;    // FP32 fast kernels: largest radix 2^rm
 };
 
@@ -449,7 +456,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     if (p.fast == 1) P = fast_strided_tile(p.a.logn);
     if (p.fast == 2) P = fast_c2r_rows(p.a.logn, p.rm);
     if (p.fast == 3) P = 256;
-    if (p.fast == 4 || p.fast == 5) P = fast64_tile(p.fast, p.a.logn);
+    if (p.fast >= 4) P = fast64_tile(p.fast, p.a.logn);
     if (p.fast == 4) {
         p.nblocks = fast64_rows_blocks(p.a.logn, p.a.npy, p.a.npz);
     } else if (p.a.axis == 0) {
@@ -490,7 +497,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
         e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, strm);
     else if (p.fast == 3)
         e = launch_fast_x1(p.nblocks, nbatch, p.a, strm);
-    else if (p.fast == 4 || p.fast == 5)
+    else if (p.fast >= 4)
         e = launch_fast64(p.fast, p.a.logn, p.out, p.nblocks, nbatch, p.a, strm);
     else
         e = launch_pass64(p.in, p.out, p.sh.nt, p.nblocks, nbatch, p.a, strm);
@@ -665,6 +672,10 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     px.a.out = L.wa.p;
     px.a.out_row = npx;
     px.a.tw = tw_ptr(c->tw64f, lx, 16);
+    // compile-time kernels (same arithmetic) where the lengths allow, as for realizations
+    const bool f64ok = !c->fp64_generic;
+    auto fits = [](int l) { return l >= 1 && l <= kMaxFast64Log; };
+    if (f64ok && (has_y || has_z) && fits(lx)) px.fast = 6;
     SS3D_TRY(run_pass(c, px, 1, L.stream, L.partial));
     Pass* last = &px;
     Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a, 1};
@@ -679,6 +690,7 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
         py.a.in_sA = py.a.out_sA = npx;
         py.a.in_sT = py.a.out_sT = (int64_t)npx * npy;
         py.a.tw = tw_ptr(c->tw64f, ly, 16);
+        if (f64ok && fits(ly)) py.fast = 5;
         SS3D_TRY(run_pass(c, py, 1, L.stream, L.partial));
         last = &py;
     }
@@ -694,6 +706,7 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
         pz.a.in_sA = pz.a.out_sA = (int64_t)npx * npy;
         pz.a.in_sT = pz.a.out_sT = npx;
         pz.a.tw = tw_ptr(c->tw64f, lz, 16);
+        if (f64ok && fits(lz)) pz.fast = 5;
         SS3D_TRY(run_pass(c, pz, 1, L.stream, L.partial));
         last = &pz;
     }

@@ -104,7 +104,8 @@ hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_
 
 // FP64 realization passes with compile-time lengths 2^1 .. 2^kMaxFast64Log (ss3d_fast64.h), bit
 // for bit the arithmetic of launch_pass64.  kind 4: x pass (fill + transform of rows, OUT_STORE),
-// kind 5: strided y/z pass (IN_LOAD, OUT_STORE or OUT_REAL).  fast64_tile: rows or columns per block.
+// kind 5: strided y/z pass (IN_LOAD, OUT_STORE, OUT_REAL or OUT_SPEC), kind 6: x pass of the
+// spectral density (IN_COV, OUT_STORE).  fast64_tile: rows or columns per block.
 constexpr int kMaxFast64Log = 12;
 int fast64_tile(int kind, int L);
 int fast64_rows_blocks(int L, int npy, int npz);  // blocks of a kind-4 launch

@@ -33,6 +33,15 @@ hipError_t f64_L(int kind, int out, int nblocks, int nbatch, const PassArgs& a, 
                               attr);
         }
     }
+    if (kind == 6) {
+        static bool attr = false;
+        return launch_f64(f64::rows_cov_kernel<L>, f64::rows_lds(L), G * f64::rows_P(L), nblocks, nbatch, a, s, attr);
+    }
+    if (out == OUT_SPEC) {
+        static bool attr = false;
+        return launch_f64(f64::strided_kernel<L, OUT_SPEC>, f64::strided_lds(L), G * f64::strided_P(L), nblocks,
+                          nbatch, a, s, attr);
+    }
     if (out == OUT_STORE) {
         static bool attr = false;
         return launch_f64(f64::strided_kernel<L, OUT_STORE>, f64::strided_lds(L), G * f64::strided_P(L), nblocks,
@@ -52,7 +61,7 @@ hipError_t f64_dispatch(int L, int kind, int out, int nblocks, int nbatch, const
 }
 }  // namespace
 
-int fast64_tile(int kind, int L) { return kind == 4 ? f64::rows_P(L) : f64::strided_P(L); }
+int fast64_tile(int kind, int L) { return kind == 4 || kind == 6 ? f64::rows_P(L) : f64::strided_P(L); }
 
 // blocks of the x pass: row-pair kernel (lengths whose rows fit one wave) or one row per unit
 int fast64_rows_blocks(int L, int npy, int npz) {
@@ -67,7 +76,8 @@ int fast64_rows_blocks(int L, int npy, int npz) {
 
 hipError_t launch_fast64(int kind, int L, int out, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     if (L < 1 || L > kMaxFast64Log) return hipErrorInvalidValue;
-    if (kind == 5 && out != OUT_STORE && out != OUT_REAL) return hipErrorInvalidValue;
+    if (kind == 5 && out != OUT_STORE && out != OUT_REAL && out != OUT_SPEC) return hipErrorInvalidValue;
+    if (kind == 6 && out != OUT_STORE) return hipErrorInvalidValue;
     return f64_dispatch(L, kind, out, nblocks, nbatch, a, s, std::make_integer_sequence<int, kMaxFast64Log>{});
 }
 
