@@ -153,6 +153,23 @@ __device__ __forceinline__ f2 coef_t(float amps, uint32_t st, bool conj, bool se
     }
     return mk(re, conj ? -im : im);
 }
+// The same for a site that is not self-conjugate, with the conjugation carried by the sign of
+// the amplitude (negative: conjugate): the real part takes |amplitude|, the imaginary part the
+// signed one -- the values of coef_t(|amps|, st, amps < 0, false), without a select.
+template <int IG>
+__device__ __forceinline__ f2 coef_signed(float amps, uint32_t st) {
+    const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
+    if constexpr (IG == 0) {
+        const float amp = amps * 1.2011224087864498f;
+        const float u = (float)st * inv_m;
+        return mk(fabsf(amp) * __builtin_amdgcn_cosf(u), amp * __builtin_amdgcn_sinf(u));
+    } else {
+        const float u1 = (float)st * inv_m;
+        const float u2 = (float)__builtin_amdgcn_fract((double)st * (16807.0 / 2147483647.0));
+        const float a = amps * __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+        return mk(fabsf(a) * __builtin_amdgcn_cosf(u2), a * __builtin_amdgcn_sinf(u2));
+    }
+}
 __device__ __forceinline__ f2 coef(float amp, uint32_t st, int igauss, bool conj, bool selfc) {
     return igauss ? coef_t<1>(amp, st, conj, selfc) : coef_t<0>(amp, st, conj, selfc);
 }
@@ -287,8 +304,8 @@ __device__ __forceinline__ float4 pack2(f2 a, f2 b) { return make_float4(a.x, a.
 // (which keeps the compiler's vector-memory counter waits exact across the member loop).
 using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
 constexpr uint32_t kOob = 0x80000000u;
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, int bytes = 0x7fffffff) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
 }
 __device__ __forceinline__ float4 bload16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
     const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
@@ -310,7 +327,7 @@ __device__ __forceinline__ void st_nt16(float4 v, float4* p) {
 // site's row factor and column factor, the conjugation and self-conjugate flags -- is read and
 // computed once and kept in registers; per realization only its base factor is multiplied in
 // (the Park-Miller state is a product mod 2^31-1, so the order of the factors does not matter).
-template <int L, int P, int IN, int AXIS, int RM>
+template <int L, int P, int IN, int AXIS, int RM, int IG>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(PassArgs a) {
     using C = StridedCfg<L, P, RM>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, PH = P / 2;
@@ -337,7 +354,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
 
     // radix passes (LDS exchange between them), then the store of batch member b:
     // butterfly g + G*i, element r at out_pos; positions >= n_out pruned
-    auto transform_store = [&](int b, const int g, const int q) {
+    auto transform_store = [&](int b, const int g, const int q) __attribute__((always_inline)) {
         [&]<int... J>(std::integer_sequence<int, J...>) {
             (
                 [&] {
@@ -384,16 +401,40 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
                     const uint32_t off = (active && pos < nout) ? (q + (uint32_t)pos * sA) * 16u : kOob;
                     bstore16(pack2(v0[e], v1[e]), rs, off);
                 }
-        } else if (active) {
-            float4* __restrict__ dst = reinterpret_cast<float4*>(const_cast<f2*>(obase));
+        } else {
+            // fill passes: element r of butterfly i sits at position pos(i, 0) + r 2^NSL.  When the
+            // axis spans less than 2 GiB the stores go through a buffer resource that ends at
+            // position nout (pruned positions fall outside it; inactive lanes use an out-of-range
+            // offset); a store whose positions are pruned for the whole wave is not issued (the
+            // positions grow with the lane, so lane 0 holds the wave's smallest)
+            constexpr int NSLl = nslog(L, M - 1, RM);
+            const uint32_t step = sA << NSLl;  // float4 units from element r to r + 1
+            if ((int64_t)N * a.out_sA * 8 <= 0x7fffffffLL) {
+                const __amdgpu_buffer_rsrc_t rs = rsrc(obase, (int)((int64_t)nout * a.out_sA * 8));
 #pragma unroll
-            for (int i = 0; i < NBl; ++i)
+                for (int i = 0; i < NBl; ++i) {
+                    const int pos0 = out_pos<L, M - 1, RM>(g + G * i, 0);
+                    const int wpos0 = __builtin_amdgcn_readfirstlane(pos0);
+                    const uint32_t o0 = active ? (q + (uint32_t)pos0 * sA) * 16u : kOob;
 #pragma unroll
-                for (int r = 0; r < (1 << LRl); ++r) {
-                    const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
-                    const int e = i * (1 << LRl) + r;
-                    if (pos < nout) st_nt16(pack2(v0[e], v1[e]), dst + q + (uint32_t)pos * sA);
+                    for (int r = 0; r < (1 << LRl); ++r) {
+                        const int e = i * (1 << LRl) + r;
+                        if (wpos0 + (r << NSLl) < nout) bstore16(pack2(v0[e], v1[e]), rs, o0 + r * step * 16u);
+                    }
                 }
+            } else if (active) {
+                float4* __restrict__ dst = reinterpret_cast<float4*>(const_cast<f2*>(obase)) + q;
+#pragma unroll
+                for (int i = 0; i < NBl; ++i) {
+                    const int pos0 = out_pos<L, M - 1, RM>(g + G * i, 0);
+                    float4* __restrict__ d0 = dst + (int64_t)pos0 * sA;
+#pragma unroll
+                    for (int r = 0; r < (1 << LRl); ++r) {
+                        const int e = i * (1 << LRl) + r;
+                        if (pos0 + (r << NSLl) < nout) st_nt16(pack2(v0[e], v1[e]), d0 + (int64_t)r * step);
+                    }
+                }
+            }
         }
     };
 
@@ -439,12 +480,13 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
         }
     } else {  // IN_FILL32H: AXIS 2: (ky, kz) = (third, pos), AXIS 1: (pos, third)
         const int b0 = blockIdx.y * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
-        // Per element and column: row factor x column factor of the site, bit 31 set when the
-        // site is the partner (conjugate).  A site is self-conjugate only in column 0 (kx = 0,
-        // and the Nyquist column it carries) on a self row (row == partner row): selfm bit r.
+        // Per element and column: row factor x column factor of the site, and the amplitude,
+        // negated when the site is the partner (conjugate).  A site is self-conjugate only in
+        // column 0 (kx = 0, and the Nyquist column it carries) on a self row (row == partner
+        // row): selfm bit r; conjm bit r: column 0's site is the partner.
         uint32_t w0[EPT], w1[EPT];
         float2 ampv[EPT];
-        uint32_t selfm = 0;
+        uint32_t selfm = 0, conjm = 0;
         const bool col0 = active && c0 == 0, pack = col0 && a.npx >= 2;
         const int kxn = a.npx >> 1;
         uint32_t txn = 0;
@@ -464,48 +506,54 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
                 const int pos = g + r * G;
                 const Site sv = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
                 const uint32_t rf = __ldg(a.rowf + sv.site);
-                ampv[r] = __ldg(reinterpret_cast<const float2*>(abase + r * astep));
+                const float2 am = __ldg(reinterpret_cast<const float2*>(abase + r * astep));
                 // the site row is shared by every column; x order decides only inside self rows
                 const bool ja = sv.gt || (sv.eq && kxa > pa);
                 const bool jb = sv.gt || (sv.eq && kxb > pb);
-                w0[r] = pm_mul(rf, ja ? txpa : txa) | uint32_t(ja) << 31;
-                w1[r] = pm_mul(rf, jb ? txpb : txb) | uint32_t(jb) << 31;
+                w0[r] = pm_mul(rf, ja ? txpa : txa);
+                w1[r] = pm_mul(rf, jb ? txpb : txb);
+                ampv[r] = make_float2(ja ? -am.x : am.x, jb ? -am.y : am.y);
                 if (sv.eq) selfm |= 1u << r;
+                if (ja) conjm |= 1u << r;
+            }
+        } else {
+            // inactive lanes: amplitude 0 and a valid state (coefficient 0), so that the member
+            // loop runs without a branch on the lane's column
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+                w0[r] = w1[r] = 1u;
+                ampv[r] = make_float2(0.f, 0.f);
             }
         }
+        // the member loop, straight-line for the coefficient law IG (igauss) of the instantiation
         for (int b = b0; b < b1; ++b) {
             if (b > b0) __syncthreads();  // LDS reads of the previous member are done
             // realization base: a uniform read through the scalar cache (written by an earlier
             // kernel, never by this one), so that no vector-memory wait (which would also wait
             // for the stores of the previous member) is needed
             const uint32_t rb = ((const __attribute__((address_space(4))) uint32_t*)a.rbase)[b];
-            if (active) {
+#pragma unroll
+            for (int r = 0; r < EPT; ++r) {
+                // keep the site values opaque so that nothing derived from them is hoisted
+                // out of the member loop (registers)
+                asm volatile("" : "+v"(w0[r]), "+v"(w1[r]), "+v"(ampv[r].x), "+v"(ampv[r].y));
+                v0[r] = coef_signed<IG>(ampv[r].x, pm_mul(w0[r], rb));
+                v1[r] = coef_signed<IG>(ampv[r].y, pm_mul(w1[r], rb));
+            }
+            if (col0) {  // kx = 0: self-conjugate sites; the Nyquist column (kx = npx/2, its own partner)
+                int zero = 0;  // opaque: the addresses below are not kept across the loop
+                asm volatile("" : "+v"(zero));
+                const float* __restrict__ ab = abase + zero;
 #pragma unroll
                 for (int r = 0; r < EPT; ++r) {
-                    // keep the site values opaque so that nothing derived from them is hoisted
-                    // out of the member loop (registers)
-                    asm volatile("" : "+v"(w0[r]), "+v"(w1[r]), "+v"(ampv[r].x), "+v"(ampv[r].y));
-                    v0[r] = coef(ampv[r].x, pm_mul(w0[r] & 0x7fffffffu, rb), a.igauss, w0[r] >> 31, false);
-                    v1[r] = coef(ampv[r].y, pm_mul(w1[r] & 0x7fffffffu, rb), a.igauss, w1[r] >> 31, false);
-                }
-                if (col0) {  // kx = 0: self-conjugate sites; the Nyquist column (kx = npx/2, its own partner)
-                    int zero = 0;  // opaque: the addresses below are not kept across the loop
-                    asm volatile("" : "+v"(zero));
-                    const float* __restrict__ ab = abase + zero;
-#pragma unroll
-                    for (int r = 0; r < EPT; ++r) {
-                        const uint32_t st = pm_mul(w0[r] & 0x7fffffffu, rb);  // column factor of kx = 0 is 1
-                        const bool cj = w0[r] >> 31, sf = (selfm >> r) & 1u;
-                        if (sf) v0[r] = coef(ampv[r].x, st, a.igauss, false, true);
-                        if (pack) {
-                            const f2 zn = coef(__ldg(ab + r * astep + kxn), pm_mul(st, txn), a.igauss, cj, sf);
-                            v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
-                        }
+                    const uint32_t st = pm_mul(w0[r], rb);  // column factor of kx = 0 is 1
+                    const bool cj = (conjm >> r) & 1u, sf = (selfm >> r) & 1u;
+                    if (sf) v0[r] = coef_t<IG>(fabsf(ampv[r].x), st, false, true);
+                    if (pack) {
+                        const f2 zn = coef_t<IG>(__ldg(ab + r * astep + kxn), pm_mul(st, txn), cj, sf);
+                        v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
                     }
                 }
-            } else {
-#pragma unroll
-                for (int r = 0; r < EPT; ++r) v0[r] = v1[r] = mk(0.f, 0.f);
             }
             // per-member recomputation of the thread's LDS, twiddle and store addresses (opaque
             // thread indices) instead of keeping them in registers across the member loop

@@ -20,11 +20,11 @@ hipError_t launch_k(K k, size_t lds, int nt, int nblocks, int nbatch, const Pass
     return hipGetLastError();
 }
 
-template <int L, int IN, int AXIS, int RM>
+template <int L, int IN, int AXIS, int RM, int IG>
 hipError_t strided_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using C = fast::StridedCfg<L, strided_P(L), RM>;
     static bool attr = false;
-    return launch_k(fast::strided_kernel<L, strided_P(L), IN, AXIS, RM>, C::lds_bytes(), C::NT,
+    return launch_k(fast::strided_kernel<L, strided_P(L), IN, AXIS, RM, IG>, C::lds_bytes(), C::NT,
                     nblocks, nbatch, a, s, attr);
 }
 
@@ -35,11 +35,11 @@ hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     return launch_k(fast::c2r_kernel<L, IN, RM>, C::lds_elems() * sizeof(float2), C::NT, nblocks, nbatch, a, s, attr);
 }
 
-template <int RM, int IN, int AXIS, int... Ls>
+template <int RM, int IN, int AXIS, int IG, int... Ls>
 hipError_t strided_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
                             std::integer_sequence<int, Ls...>) {
     hipError_t e = hipErrorInvalidValue;
-    ((L == Ls + 1 ? (e = strided_L<Ls + 1, IN, AXIS, RM>(nblocks, nbatch, a, s), 0) : 0), ...);
+    ((L == Ls + 1 ? (e = strided_L<Ls + 1, IN, AXIS, RM, IG>(nblocks, nbatch, a, s), 0) : 0), ...);
     return e;
 }
 
@@ -54,9 +54,14 @@ hipError_t c2r_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipSt
 template <int RM>
 hipError_t strided_rm(int L, int in_mode, int axis, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using seq = std::make_integer_sequence<int, kMaxFastLog>;
-    if (in_mode == IN_LOAD) return strided_dispatch<RM, IN_LOAD, 1>(L, nblocks, nbatch, a, s, seq{});
-    if (in_mode == IN_FILL32H && axis == 2) return strided_dispatch<RM, IN_FILL32H, 2>(L, nblocks, nbatch, a, s, seq{});
-    if (in_mode == IN_FILL32H && axis == 1) return strided_dispatch<RM, IN_FILL32H, 1>(L, nblocks, nbatch, a, s, seq{});
+    // fill passes: one instantiation per coefficient law (igauss 0 or 1)
+    if (in_mode == IN_LOAD) return strided_dispatch<RM, IN_LOAD, 1, 0>(L, nblocks, nbatch, a, s, seq{});
+    if (in_mode == IN_FILL32H && axis == 2)
+        return a.igauss ? strided_dispatch<RM, IN_FILL32H, 2, 1>(L, nblocks, nbatch, a, s, seq{})
+                        : strided_dispatch<RM, IN_FILL32H, 2, 0>(L, nblocks, nbatch, a, s, seq{});
+    if (in_mode == IN_FILL32H && axis == 1)
+        return a.igauss ? strided_dispatch<RM, IN_FILL32H, 1, 1>(L, nblocks, nbatch, a, s, seq{})
+                        : strided_dispatch<RM, IN_FILL32H, 1, 0>(L, nblocks, nbatch, a, s, seq{});
     return hipErrorInvalidValue;
 }
 
