@@ -126,11 +126,11 @@ int ss3d_realize_device(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed,
 
 /* Per-pass timing with HIP events recorded on the context stream around every pass launch.
  * Pass kinds: 0..2 spectrum x/y/z (FP64), 3..5 FP64 realization x/y/z, 6..8 FP32 realization
- * z/y/x.  enable=1 clears and starts recording, enable=0 stops.  profile_read synchronises the
+ * z/y/x, 9 FP32 z fill beside the y pass of the previous batch (SS3D_OPT_FUSE).  enable=1 clears and starts recording, enable=0 stops.  profile_read synchronises the
  * stream and returns, per kind, total milliseconds, algorithmic HBM bytes (bytes each pass must
  * read and write) and launch count.  No counterpart in the reference (its timers are the wall
  * clocks printed at f90:1299-1300, 1350-1351). */
-#define SS3D_NPASS_KINDS 9
+#define SS3D_NPASS_KINDS 10
 int ss3d_context_profile(ss3d_context* ctx, int enable);
 int ss3d_context_profile_read(ss3d_context* ctx, double* ms, double* bytes, int64_t* launches, int nkinds);
 
@@ -150,6 +150,10 @@ int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
 /* SS3D_OPT_FP64_GENERIC (default 0): FP64 realizations through the generic (runtime-length) pass
  * kernel instead of the compile-time kernels; both compute the same values bit for bit. */
 #define SS3D_OPT_FP64_GENERIC 2
+/* SS3D_OPT_FUSE (default 1): FP32 realizations with z and y axes of one length run the fill pass
+ * of batch k+1 and the y pass of batch k in one launch whose blocks interleave (arithmetic-bound
+ * and memory-bound tiles share each CU); 0 runs them as separate launches.  Same values. */
+#define SS3D_OPT_FUSE 3
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
 /* ---------------------------------------------------------------- request loop

@@ -22,6 +22,7 @@ FP64 = 0
 FP32 = 1
 OPT_PIPELINE = 1
 OPT_FP64_GENERIC = 2
+OPT_FUSE = 3
 
 ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device", 6: "I/O error"}
 
@@ -169,9 +170,9 @@ SIGNATURES = {
     "ss3d_format_es": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
 }
 
-NPASS_KINDS = 9
+NPASS_KINDS = 10
 PASS_KINDS = ["spectrum_x_fp64", "spectrum_y_fp64", "spectrum_z_fp64", "real_x_fp64", "real_y_fp64", "real_z_fp64",
-              "real_z_fp32", "real_y_fp32", "real_x_c2r_fp32"]
+              "real_z_fp32", "real_y_fp32", "real_x_c2r_fp32", "real_zy_fp32"]
 
 
 def lib():

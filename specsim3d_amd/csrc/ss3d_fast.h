@@ -327,8 +327,9 @@ __device__ __forceinline__ void st_nt16(float4 v, float4* p) {
 // site's row factor and column factor, the conjugation and self-conjugate flags -- is read and
 // computed once and kept in registers; per realization only its base factor is multiplied in
 // (the Park-Miller state is a product mod 2^31-1, so the order of the factors does not matter).
+// One tile of a strided pass (the work of one block): `tile` in XCD order, member group `grp`.
 template <int L, int P, int IN, int AXIS, int RM, int IG>
-__global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(PassArgs a) {
+__device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, const int grp) {
     using C = StridedCfg<L, P, RM>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, PH = P / 2;
     static_assert(M > 0, "strided passes have length >= 2");
@@ -345,7 +346,6 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
         if constexpr (C::TWL) return TwLds{twl};
         else return TwGlobal{tw};
     }();
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int i0base = (tile % a.ntiles0) * P;
     const int third = tile / a.ntiles0;
     const int c0 = i0base + 2 * q;
@@ -441,7 +441,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
     if constexpr (IN == IN_LOAD) {  // pass-0 operands at positions g + r*G
         // members b0 .. b1-1 of this tile; the operands of member b+1 are loaded while member b
         // is transformed and stored
-        const int b0 = blockIdx.y * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
+        const int b0 = grp * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
         const uint32_t sA = (uint32_t)a.in_sA >> 1;  // in float4 units
         const uint32_t o0 = q + (uint32_t)g * sA, step = (uint32_t)G * sA;
         float4 nxt[EPT];
@@ -479,7 +479,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
             __syncthreads();  // LDS reads of member b are done before the exchange of b+1
         }
     } else {  // IN_FILL32H: AXIS 2: (ky, kz) = (third, pos), AXIS 1: (pos, third)
-        const int b0 = blockIdx.y * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
+        const int b0 = grp * a.bpb, b1 = min(b0 + a.bpb, a.nbatch);
         // Per element and column: row factor x column factor of the site, and the amplitude,
         // negated when the site is the partner (conjugate).  A site is self-conjugate only in
         // column 0 (kx = 0, and the Nyquist column it carries) on a self row (row == partner
@@ -562,6 +562,37 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
             transform_store(b, gg, qq);
         }
     }
+}
+
+template <int L, int P, int IN, int AXIS, int RM, int IG>
+__global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(PassArgs a) {
+    strided_tile<L, P, IN, AXIS, RM, IG>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.y);
+}
+
+// ------------------------------------------------------------ fill pass beside a y pass
+// The z fill pass of one batch and the y pass of the previous batch (same length and tile) in
+// one launch.  Block roles interleave -- 8A fill blocks, then 8C y blocks, K times, then the
+// rest of each -- so that every CU holds arithmetic-bound fill tiles and memory-bound y tiles
+// at once.  Role-local block l runs tile l mod nblocks (in XCD order) of member group
+// l / nblocks of its pass; 8A and 8C keep l and the block index equal mod 8 (the same XCD).
+template <int L, int P, int RM, int IG>
+__global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(PassArgs az, PassArgs ay, FuseMap m) {
+    const int per = 8 * (m.A + m.C), bid = blockIdx.x, full = m.K * per;
+    bool fill;
+    int l;
+    if (bid < full) {
+        const int pr = bid / per, o = bid - pr * per;
+        fill = o < 8 * m.A;
+        l = fill ? pr * 8 * m.A + o : pr * 8 * m.C + (o - 8 * m.A);
+    } else {
+        const int o = bid - full, zr = m.nzb - 8 * m.A * m.K;
+        fill = o < zr;
+        l = fill ? 8 * m.A * m.K + o : 8 * m.C * m.K + (o - zr);
+    }
+    if (fill)
+        strided_tile<L, P, IN_FILL32H, 2, RM, IG>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
+    else
+        strided_tile<L, P, IN_LOAD, 1, RM, 0>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
 }
 
 // ------------------------------------------------------------ C2R pass along x

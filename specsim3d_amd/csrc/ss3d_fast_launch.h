@@ -28,6 +28,38 @@ hipError_t strided_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) 
                     nblocks, nbatch, a, s, attr);
 }
 
+template <int L, int RM, int IG>
+hipError_t fused_L(int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    using C = fast::StridedCfg<L, strided_P(L), RM>;
+    static bool attr = false;
+    const auto k = fast::fill_load_kernel<L, strided_P(L), RM, IG>;
+    const size_t lds = C::lds_bytes();
+    if (lds > 65536 && !attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(C::NT), lds, s, az, ay, m);
+    return hipGetLastError();
+}
+
+template <int RM, int IG, int... Ls>
+hipError_t fused_dispatch(int L, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
+                          hipStream_t s, std::integer_sequence<int, Ls...>) {
+    hipError_t e = hipErrorInvalidValue;
+    ((L == Ls + 1 ? (e = fused_L<Ls + 1, RM, IG>(nblocks, az, ay, m, s), 0) : 0), ...);
+    return e;
+}
+
+// z fill (az) beside the y pass (ay), both of length 2^L, in nblocks = m.nzb + m.nyb blocks
+template <int RM>
+hipError_t fused_rm(int L, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    static_assert(2 * sizeof(PassArgs) + sizeof(FuseMap) <= 4096, "kernel arguments");
+    using seq = std::make_integer_sequence<int, kMaxFastLog>;
+    return az.igauss ? fused_dispatch<RM, 1>(L, nblocks, az, ay, m, s, seq{})
+                     : fused_dispatch<RM, 0>(L, nblocks, az, ay, m, s, seq{});
+}
+
 template <int L, int IN, int RM>
 hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using C = fast::C2RCfg<L, RM>;

@@ -230,6 +230,7 @@ struct ss3d_context {
     DevBuf tw32n;
     Lane main;               // the context's own lane (main.stream is `stream`)
     bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
+    bool fuse = true;        // SS3D_OPT_FUSE: FP32 fill of batch k+1 and y pass of batch k in one launch
     bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
@@ -450,7 +451,9 @@ hipEvent_t take_event(ss3d_context* c) {
     return e;
 }
 
-int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& partial) {
+// Tiles, blocks and member groups of a launch of p over nbatch members (set in p.a), the
+// statistics buffer of passes that write statistics; `groups` = block groups along members.
+int prep_pass(Pass& p, int nbatch, hipStream_t strm, DevBuf& partial, int& groups) {
     p.a.logp = p.sh.logp;
     int P = 1 << p.sh.logp;
     if (p.fast == 1) P = fast_strided_tile(p.a.logn);
@@ -472,7 +475,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     p.a.bpb = 1;
     if (p.fast == 1)
         p.a.bpb = std::max(1, std::min(nbatch, (int)((int64_t)p.nblocks * nbatch / kFillBlocks)));
-    const int groups = (nbatch + p.a.bpb - 1) / p.a.bpb;
+    groups = (nbatch + p.a.bpb - 1) / p.a.bpb;
     if (p.fast == 1 && p.in == IN_LOAD) {  // buffer addressing: byte offsets inside a tile below 2^31
         const int64_t span_in = ((int64_t(1) << p.a.logn) * p.a.in_sA + P) * 8;
         const int64_t span_out = ((int64_t)p.a.n_out * p.a.out_sA + P) * 8;
@@ -483,6 +486,12 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
         SS3D_TRY(partial.ensure((size_t)p.nblocks * nbatch * 4 * sizeof(double), strm));
         p.a.partial = partial.as<double>();
     }
+    return SS3D_OK;
+}
+
+int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& partial) {
+    int groups = 1;
+    SS3D_TRY(prep_pass(p, nbatch, strm, partial, groups));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->profiling) {
         e0 = take_event(c);
@@ -506,6 +515,36 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     if (c->profiling) {
         HIP_TRY(hipEventRecord(e1, strm));
         c->prof.push_back({p.kind, e0, e1, pass_bytes(p, nbatch, groups)});
+    }
+    return SS3D_OK;
+}
+
+// The fill pass pz over nbz members beside the y pass py over nby members, in one launch of
+// the fused kernel (pass kind 9).  Block roles interleave in the ratio of their block counts.
+int run_fused(ss3d_context* c, Pass& pz, int nbz, Pass& py, int nby, hipStream_t strm, DevBuf& partial) {
+    int gz = 1, gy = 1;
+    SS3D_TRY(prep_pass(pz, nbz, strm, partial, gz));
+    SS3D_TRY(prep_pass(py, nby, strm, partial, gy));
+    const int64_t nzb = (int64_t)pz.nblocks * gz, nyb = (int64_t)py.nblocks * gy;
+    if (nzb + nyb > 0x7fffffffLL) return fail(SS3D_ETOOBIG, "fused pass of %lld blocks", (long long)(nzb + nyb));
+    FuseMap m;
+    m.nzb = (int)nzb;
+    m.nyb = (int)nyb;
+    m.A = nzb >= nyb ? (int)std::max<int64_t>(1, (nzb + nyb / 2) / nyb) : 1;
+    m.C = nzb >= nyb ? 1 : (int)std::max<int64_t>(1, (nyb + nzb / 2) / nzb);
+    m.K = std::min(m.nzb / (8 * m.A), m.nyb / (8 * m.C));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profiling) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (!e0 || !e1) return fail(SS3D_EHIP, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(e0, strm));
+    }
+    hipError_t e = launch_fast_fused(pz.a.logn, pz.rm, pz.a, py.a, m, strm);
+    if (e != hipSuccess) return fail(SS3D_EHIP, "fused pass launch: %s", hipGetErrorString(e));
+    if (c->profiling) {
+        HIP_TRY(hipEventRecord(e1, strm));
+        c->prof.push_back({9, e0, e1, pass_bytes(pz, nbz, gz) + pass_bytes(py, nby, gy)});
     }
     return SS3D_OK;
 }
@@ -1012,11 +1051,43 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
         for (int i = 0; i < 2; ++i) SS3D_TRY(ensure2(L, L.pipe[i], (size_t)std::max<int64_t>(pipe_elems, 1) * nb * 8));
         if (has_z && has_y) SS3D_TRY(ensure2(L, L.wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
     }
+    const int nbat = (nreal + nb - 1) / nb;
+    if (staged && has_z && has_y && !pipelined && c->fuse && lz == ly && nbat >= 2) {
+        // launch k: fill of batch k beside the y pass of batch k-1 (pipe buffers alternate),
+        // then the x pass of batch k-1
+        py.rm = pz.rm;  // one radix plan for both roles of the fused kernel
+        for (int k = 0; k <= nbat; ++k) {
+            const bool zk = k < nbat, yk = k >= 1;
+            const int cz = zk ? std::min(nb, nreal - k * nb) : 0;
+            const int p0 = (k - 1) * nb, cy = yk ? std::min(nb, nreal - p0) : 0;
+            if (zk) {
+                pz.a.rbase = L.rbase.as<uint32_t>() + (size_t)k * nb;
+                pz.a.out = L.pipe[k & 1].p;
+            }
+            if (yk) {
+                py.a.in = L.pipe[(k - 1) & 1].p;
+                py.a.out = L.wb.p;
+            }
+            if (zk && yk)
+                SS3D_TRY(run_fused(c, pz, cz, py, cy, L.stream, L.partial));
+            else if (zk)
+                SS3D_TRY(run_pass(c, pz, cz, L.stream, L.partial));
+            else
+                SS3D_TRY(run_pass(c, py, cy, L.stream, L.partial));
+            if (!yk) continue;
+            px.a.in = L.wb.p;
+            px.a.out = static_cast<char*>(dout) + (size_t)p0 * ncrop * 4;
+            SS3D_TRY(run_pass(c, px, cy, L.stream, L.partial));
+            hipError_t e = launch_stats_finalize(L.partial.as<double>(), px.nblocks, cy, (double)ncrop,
+                                                 dstats + 3 * (size_t)p0, L.stream);
+            if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
+        }
+        return SS3D_OK;
+    }
     if (pipelined) {
         HIP_TRY(hipEventRecord(L.ev_join, L.stream));
         HIP_TRY(hipStreamWaitEvent(L.stream2, L.ev_join, 0));
     }
-    const int nbat = (nreal + nb - 1) / nb;
     for (int k = 0; k < nbat; ++k) {
         const int done = k * nb, cnt = std::min(nb, nreal - done);
         const int buf = k & 1;
@@ -1197,6 +1268,10 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
     }
     if (option == SS3D_OPT_FP64_GENERIC) {
         c->fp64_generic = value != 0;
+        return SS3D_OK;
+    }
+    if (option == SS3D_OPT_FUSE) {
+        c->fuse = value != 0;
         return SS3D_OK;
     }
     return fail(SS3D_EINVAL, "unknown option %d", option);

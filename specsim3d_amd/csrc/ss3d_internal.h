@@ -101,6 +101,14 @@ hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int nblocks
                                hipStream_t s);
 hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
+// z fill pass (az) beside the y pass (ay) of the previous batch in one launch of
+// m.nzb + m.nyb blocks (both passes of length 2^L, largest radix 2^rm); the block roles
+// interleave as 8A fill blocks, 8C y blocks, K times, then the rest of each.
+struct FuseMap {
+    int nzb, nyb;  // blocks of the fill and of the y pass (tiles x member groups)
+    int A, C, K;
+};
+hipError_t launch_fast_fused(int L, int rm, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s);
 
 // FP64 realization passes with compile-time lengths 2^1 .. 2^kMaxFast64Log (ss3d_fast64.h), bit
 // for bit the arithmetic of launch_pass64.  kind 4: x pass (fill + transform of rows, OUT_STORE),

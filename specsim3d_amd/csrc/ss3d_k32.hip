@@ -6,6 +6,9 @@ namespace ss3d {
 hipError_t launch_fast_strided_r3(int, int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_strided_r4(int, int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_strided_r5(int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_fused_r3(int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
+hipError_t launch_fast_fused_r4(int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
+hipError_t launch_fast_fused_r5(int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
 hipError_t launch_fast_c2r_r3(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r4(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r5(int, int, int, int, const PassArgs&, hipStream_t);
@@ -35,6 +38,16 @@ hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, 
         case 3: return launch_fast_c2r_r3(L, in_mode, nblocks, nbatch, a, s);
         case 4: return launch_fast_c2r_r4(L, in_mode, nblocks, nbatch, a, s);
         case 5: return launch_fast_c2r_r5(L, in_mode, nblocks, nbatch, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fast_fused(int L, int rm, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    if (L < 1 || L > kMaxFastLog) return hipErrorInvalidValue;
+    switch (rm) {
+        case 3: return launch_fast_fused_r3(L, az, ay, m, s);
+        case 4: return launch_fast_fused_r4(L, az, ay, m, s);
+        case 5: return launch_fast_fused_r5(L, az, ay, m, s);
         default: return hipErrorInvalidValue;
     }
 }

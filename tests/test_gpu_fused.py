@@ -1,0 +1,77 @@
+"""The fused FP32 schedule (SS3D_OPT_FUSE, on by default): the z fill pass of batch k+1 and the y
+pass of batch k run in one launch whose blocks interleave.  The launch performs the arithmetic
+of the separate passes on the same tiles, so the fields and statistics equal those of separate
+launches bit for bit.  The cases exercise the block-role map: more fill blocks than y blocks and
+fewer (a short last batch), a tail of one role, lengths 2^5 .. 2^7, both coefficient laws, and a
+grid whose z and y lengths differ (no fusion: separate launches either way).  The fused pass is
+what runs: pass kind 9 appears in the profile only with the option on.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import specsim3d_amd as S
+
+pytestmark = pytest.mark.gpu
+
+# (structures, nugget, output grid, realizations, igauss, fused) -- periodic grids 128^3 (8 per
+# batch: 8, 8, 3), 64^3 (64 per batch: 64, 6), 128x64x64 (32, 8), 32^3 (512, 88), 256x256x64
+CASES = [
+    ([(2, 0.7, 20.0, 10.0, 5.0, 30.0, 15.0, 0.0)], 0.3, (64, 64, 64), 19, 1, True),
+    ([(1, 1.0, 15.0, 15.0, 15.0, 0.0, 0.0, 0.0)], 0.0, (48, 40, 40), 70, 1, True),
+    ([(3, 1.0, 6.0, 6.0, 6.0, 0.0, 0.0, 0.0)], 0.0, (60, 30, 30), 40, 0, True),
+    ([(2, 1.0, 3.0, 3.0, 3.0, 0.0, 0.0, 0.0)], 0.0, (16, 16, 16), 600, 1, True),
+    ([(1, 1.0, 30.0, 30.0, 30.0, 0.0, 0.0, 0.0)], 0.0, (100, 100, 20), 17, 1, False),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    with S.Engine(0) as e:
+        yield e
+
+
+@pytest.mark.parametrize("sts,nugget,grid,nreal,igauss,fused", CASES)
+def test_fused_equals_separate_launches(eng, sts, nugget, grid, nreal, igauss, fused):
+    nx, ny, nz = grid
+    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    say(f"[fused] grid {grid} periodic {npg} nreal {nreal} igauss {igauss}")
+    sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
+    try:
+        got = {}
+        for opt in (1, 0):
+            eng.set_option(S.OPT_FUSE, opt)
+            eng.profile(True)
+            got[opt] = eng.realize(sp, 777, igauss, 5, nreal, nx, ny, nz, precision=S.FP32)
+            eng.profile(False)
+            kinds = eng.profile_read()
+            assert ("real_zy_fp32" in kinds) == (fused and opt == 1), kinds
+    finally:
+        eng.set_option(S.OPT_FUSE, 1)
+        sp.close()
+    (f1, m1, v1, i1), (f0, m0, v0, i0) = got[1], got[0]
+    assert np.array_equal(f1, f0)
+    assert np.array_equal(m1, m0) and np.array_equal(v1, v0) and np.array_equal(i1, i0)
+    assert np.all(np.isfinite(f1)) and np.all(v1 > 0)
+
+
+def test_fused_matches_fp64(eng):
+    """The fused schedule's FP32 fields against the FP64 drop-in path: <= 1e-5 relative RMS."""
+    sts, nugget, (nx, ny, nz), nreal, igauss, _ = CASES[0]
+    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    say(f"[fused] fp32 vs fp64 {npg}")
+    sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
+    try:
+        f32 = eng.realize(sp, 4242, igauss, 0, nreal, nx, ny, nz, precision=S.FP32)[0]
+        f64 = eng.realize(sp, 4242, igauss, 0, nreal, nx, ny, nz, precision=S.FP64)[0]
+    finally:
+        sp.close()
+    rel = np.sqrt(np.mean((f32 - f64) ** 2) / np.mean(f64 ** 2))
+    say(f"relative RMS {rel:.2e}")
+    assert rel < 1e-5
