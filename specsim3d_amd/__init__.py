@@ -363,6 +363,10 @@ class Engine:
     def tuning_set(self, kind, logn, rm):
         check(lib().ss3d_tuning_set(self.handle, kind, logn, rm))
 
+    def tuning_get(self, kind, logn):
+        """Radix plan (log2 of the largest radix) of FP32 pass kind 6..8 at length 2^logn."""
+        return lib().ss3d_tuning_get(self.handle, kind, logn)
+
     def profile(self, enable):
         """Start (clears) or stop per-pass HIP-event timing on the context stream."""
         check(lib().ss3d_context_profile(self.handle, 1 if enable else 0))

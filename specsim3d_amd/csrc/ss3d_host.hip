@@ -1052,10 +1052,13 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
         if (has_z && has_y) SS3D_TRY(ensure2(L, L.wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
     }
     const int nbat = (nreal + nb - 1) / nb;
-    if (staged && has_z && has_y && !pipelined && c->fuse && lz == ly && nbat >= 2) {
+    // fusion only where the y pass's own radix plan is the fill's (results do not depend on
+    // the schedule)
+    if (staged && has_z && has_y && !pipelined && c->fuse && lz == ly && nbat >= 2 &&
+        fast_same_plan(ly, py.rm, pz.rm)) {
         // launch k: fill of batch k beside the y pass of batch k-1 (pipe buffers alternate),
         // then the x pass of batch k-1
-        py.rm = pz.rm;  // one radix plan for both roles of the fused kernel
+        py.rm = pz.rm;  // the same plan, one template instance for both roles
         for (int k = 0; k <= nbat; ++k) {
             const bool zk = k < nbat, yk = k >= 1;
             const int cz = zk ? std::min(nb, nreal - k * nb) : 0;

@@ -108,6 +108,7 @@ struct FuseMap {
     int nzb, nyb;  // blocks of the fill and of the y pass (tiles x member groups)
     int A, C, K;
 };
+bool fast_same_plan(int L, int rm1, int rm2);  // the FP32 radix plans of length 2^L for rm1 and rm2 are one
 hipError_t launch_fast_fused(int L, int rm, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s);
 
 // FP64 realization passes with compile-time lengths 2^1 .. 2^kMaxFast64Log (ss3d_fast64.h), bit

@@ -42,6 +42,17 @@ hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, 
     }
 }
 
+bool fast_same_plan(int L, int rm1, int rm2) {
+    auto np = [L](int rm) { return rm == 3 ? fast::npass(L, 3) : rm == 4 ? fast::npass(L, 4) : fast::npass(L, 5); };
+    auto rl = [L](int rm, int j) {
+        return rm == 3 ? fast::rlog(L, j, 3) : rm == 4 ? fast::rlog(L, j, 4) : fast::rlog(L, j, 5);
+    };
+    if (np(rm1) != np(rm2)) return false;
+    for (int j = 0; j < np(rm1); ++j)
+        if (rl(rm1, j) != rl(rm2, j)) return false;
+    return true;
+}
+
 hipError_t launch_fast_fused(int L, int rm, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     if (L < 1 || L > kMaxFastLog) return hipErrorInvalidValue;
     switch (rm) {

@@ -75,3 +75,30 @@ def test_fused_matches_fp64(eng):
     rel = np.sqrt(np.mean((f32 - f64) ** 2) / np.mean(f64 ** 2))
     say(f"relative RMS {rel:.2e}")
     assert rel < 1e-5
+
+
+def test_no_fusion_across_radix_plans(eng):
+    """A y pass whose radix plan differs from the fill's is not fused (the fused kernel runs one
+    plan): the realizations are those of the separate launches either way."""
+    sts, nugget, (nx, ny, nz), nreal, igauss, _ = CASES[0]
+    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    ly = int(np.log2(npg[1]))
+    say(f"[fused] y plan 2^3 against fill plan 2^4, length 2^{ly}")
+    old = {k: eng.tuning_get(k, ly) for k in (6, 7)}
+    sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
+    try:
+        eng.tuning_set(6, ly, 4)
+        eng.tuning_set(7, ly, 3)
+        got = {}
+        for opt in (1, 0):
+            eng.set_option(S.OPT_FUSE, opt)
+            eng.profile(True)
+            got[opt] = eng.realize(sp, 31, igauss, 0, nreal, nx, ny, nz, precision=S.FP32)[0]
+            eng.profile(False)
+            assert "real_zy_fp32" not in eng.profile_read()
+    finally:
+        eng.set_option(S.OPT_FUSE, 1)
+        for k, v in old.items():
+            eng.tuning_set(k, ly, v)
+        sp.close()
+    assert np.array_equal(got[1], got[0])
