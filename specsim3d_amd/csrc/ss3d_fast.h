@@ -347,6 +347,10 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
         else return TwGlobal{tw};
     }();
     const int i0base = (tile % a.ntiles0) * P;
+    // element offsets of the tile's first column in the input and output (column-tile strides
+    // sC: P for x-fastest buffers, the size of a column tile's slab for column-tiled ones)
+    const int64_t cin = (int64_t)(tile % a.ntiles0) * (a.in_sC ? a.in_sC : P);
+    const int64_t cout = (int64_t)(tile % a.ntiles0) * (a.out_sC ? a.out_sC : P);
     const int third = tile / a.ntiles0;
     const int c0 = i0base + 2 * q;
     const bool active = c0 < a.n0;
@@ -389,7 +393,7 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
         constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
         const uint32_t sA = (uint32_t)a.out_sA >> 1;  // in float4 units
         const int nout = a.n_out;
-        const f2* obase = reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + i0base;
+        const f2* obase = reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)third * a.out_sT + cout;
         if constexpr (IN == IN_LOAD) {  // branch-free: pruned and inactive lanes store out of range
             const __amdgpu_buffer_rsrc_t rs = rsrc(obase);
 #pragma unroll
@@ -448,7 +452,7 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
         auto load = [&](int b) {  // b >= b1: zeros, no memory access
             const int bb = min(b, b1 - 1);
             const __amdgpu_buffer_rsrc_t rs = rsrc(reinterpret_cast<const f2*>(a.in) + (int64_t)bb * a.in_bstride +
-                                                   (int64_t)third * a.in_sT + i0base);
+                                                   (int64_t)third * a.in_sT + cin);
             const bool ok = active && b < b1;
 #pragma unroll
             for (int r = 0; r < EPT; ++r) nxt[r] = bload16(rs, ok ? (o0 + r * step) * 16u : kOob);

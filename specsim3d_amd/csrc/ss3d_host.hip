@@ -1030,6 +1030,22 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     py.a.out_sT = (int64_t)hcs * ny;
     py.a.out_bstride = by;
     py.a.tw = wtab(ly);
+    // z and y of one length (one tile width P): the fill writes BZ in column tiles, a slab of
+    // P columns x npy rows per tile and z plane, so that a y tile reads one contiguous slab
+    int64_t bzs = bz;
+    if (has_z && has_y && lz == ly) {
+        const int P = fast_strided_tile(lz);
+        const int64_t slab = (int64_t)npy * P, plane = (int64_t)((hc + P - 1) / P) * slab;
+        bzs = plane * nz;
+        pz.a.out_sA = plane;
+        pz.a.out_sT = P;
+        pz.a.out_sC = slab;
+        pz.a.out_bstride = bzs;
+        py.a.in_sA = P;
+        py.a.in_sT = plane;
+        py.a.in_sC = slab;
+        py.a.in_bstride = bzs;
+    }
     // x pass: npx >= 2 -> C2R on complex length npx/2; npx = 1 -> real part of each row
     Pass px{32, (has_y || has_z) ? IN_LOADH : IN_FILLH, OUT_REAL, Shape{0, 0}, 0, a, 8, npx >= 2 ? 2 : 3};
     px.rm = tuned_rm(c, 8, npx >= 2 ? lx - 1 : 0);
@@ -1044,7 +1060,7 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     px.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
 
     const bool staged = has_y || has_z;  // a fill pass followed by further passes
-    const int64_t pipe_elems = has_z ? bz : by;  // output of the fill pass
+    const int64_t pipe_elems = has_z ? bzs : by;  // output of the fill pass
     hipStream_t s2 = c->pipeline ? L.stream2 : L.stream;  // stream of the passes after the fill
     const bool pipelined = staged && c->pipeline;
     if (staged) {

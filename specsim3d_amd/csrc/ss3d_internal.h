@@ -40,6 +40,7 @@ struct PassArgs {
     // contiguous tiles: whole rows
     int nrows;
     int64_t in_sA, in_sT, out_sA, out_sT;  // element strides (axis, third dimension)
+    int64_t in_sC, out_sC;                 // strided passes: element stride between column tiles (0: P)
     int64_t in_row, out_row;               // row strides of contiguous tiles
     int64_t in_bstride, out_bstride;       // per batch member (realization)
     const void* in;
