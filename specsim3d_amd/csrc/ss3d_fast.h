@@ -307,8 +307,9 @@ constexpr uint32_t kOob = 0x80000000u;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, int bytes = 0x7fffffff) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
 }
+// loads non-temporal (aux 2): each operand is read once (2-4% faster y pass than cached loads)
 __device__ __forceinline__ float4 bload16(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
-    const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 2);
     return make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
 }
 __device__ __forceinline__ void bstore16(float4 v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
