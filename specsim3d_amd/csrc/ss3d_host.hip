@@ -1255,7 +1255,7 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
     SS3D_TRY(ensure2(c->main, c->main.dout, (size_t)nb * ncrop * 4));
     SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)nb * 3 * sizeof(double)));
     const bool was = c->profiling;
-    double best[3] = {1e30, 1e30, 1e30};
+    double best[3] = {1e30, 1e30, 1e30}, tzy[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
     int pick[3] = {5, 5, 5};
     for (int rm = 3; rm <= 5; ++rm) {
         for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = rm;
@@ -1271,8 +1271,16 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
                 best[k] = ms[6 + k];
                 pick[k] = rm;
             }
+        if (nl[6] && nl[7]) tzy[rm] = ms[6] + ms[7];
     }
     SS3D_TRY(ss3d_context_profile(c, was ? 1 : 0));
+    // z and y of one length run fused only with one radix plan: pick the plan of both by their sum
+    if (lz == ly && sp->npy > 1 && sp->npz > 1 && !fast_same_plan(lz, pick[0], pick[1])) {
+        int rm = 3;
+        for (int r = 4; r <= 5; ++r)
+            if (tzy[r] < tzy[rm]) rm = r;
+        pick[0] = pick[1] = rm;
+    }
     for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = pick[k];
     return SS3D_OK;
 }

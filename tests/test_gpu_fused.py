@@ -102,3 +102,22 @@ def test_no_fusion_across_radix_plans(eng):
             eng.tuning_set(k, ly, v)
         sp.close()
     assert np.array_equal(got[1], got[0])
+
+
+def test_autotune_keeps_fusion(eng):
+    """The autotuner gives the fill and the y pass of one length a single radix plan, so the tuned
+    path runs fused."""
+    sts, nugget, (nx, ny, nz), nreal, igauss, _ = CASES[0]
+    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    say(f"[fused] autotune {npg}")
+    sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
+    try:
+        plan = eng.autotune(sp, nx, ny, nz, igauss, reps=1)
+        eng.profile(True)
+        eng.realize(sp, 5, igauss, 0, nreal, nx, ny, nz, precision=S.FP32)
+        eng.profile(False)
+        kinds = eng.profile_read()
+    finally:
+        sp.close()
+    say(f"plan {plan}")
+    assert "real_zy_fp32" in kinds, (plan, kinds)
