@@ -1170,9 +1170,10 @@ int batch_size(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nr
     const int64_t per = prec == SS3D_FP64
                             ? ((int64_t)nx * sp->npy * sp->npz + (int64_t)nx * ny * sp->npz) * 16
                             : ((int64_t)std::max(2, sp->npx / 2) * (sp->npy * (int64_t)nz + (int64_t)ny * nz)) * 8;
-    // 16 GiB of work buffers (of 288 GB): 8 realizations per launch up to 1024x1024x512 periodic
-    // grids, whose fill pass then amortizes its per-site work over 8 members (63 instead of 54
-    // Gcells/s for 512x512x256 in 1024x1024x512)
+    // `per` counts one fill output and one y output per member (the second fill buffer of the
+    // fused launch comes on top).  16 GiB of it (of 288 GB): 8 realizations per launch up to
+    // 1024x1024x512 periodic grids, whose fill pass then amortizes its per-site work over 8
+    // members (63 instead of 54 Gcells/s for 512x512x256 in 1024x1024x512)
     const int64_t budget = int64_t(16) << 30;
     int64_t nb = std::max<int64_t>(1, budget / std::max<int64_t>(per, 1));
     // small grids: enough realizations per launch to fill the GPU; large grids: up to 8
