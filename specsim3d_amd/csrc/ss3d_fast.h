@@ -576,13 +576,15 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
 }
 
 // ------------------------------------------------------------ fill pass beside a y pass
-// The z fill pass of one batch and the y pass of the previous batch (same length and tile) in
-// one launch.  Block roles interleave -- 8A fill blocks, then 8C y blocks, K times, then the
-// rest of each -- so that every CU holds arithmetic-bound fill tiles and memory-bound y tiles
-// at once.  Role-local block l runs tile l mod nblocks (in XCD order) of member group
-// l / nblocks of its pass; 8A and 8C keep l and the block index equal mod 8 (the same XCD).
-template <int L, int P, int RM, int IG>
+// The z fill pass of one batch (length 2^L, tile P, plan RM) and the y pass of the previous batch
+// (length 2^LY, tile PY, plan RMY; tiles of the same thread count) in one launch.  Block roles
+// interleave -- 8A fill blocks, then 8C y blocks, K times, then the rest of each -- so that every
+// CU holds arithmetic-bound fill tiles and memory-bound y tiles at once.  Role-local block l runs
+// tile l mod nblocks (in XCD order) of member group l / nblocks of its pass; 8A and 8C keep l and
+// the block index equal mod 8 (the same XCD).
+template <int L, int P, int RM, int IG, int LY = L, int PY = P, int RMY = RM>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(PassArgs az, PassArgs ay, FuseMap m) {
+    static_assert(StridedCfg<LY, PY, RMY>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
     const int per = 8 * (m.A + m.C), bid = blockIdx.x, full = m.K * per;
     bool fill;
     int l;
@@ -598,7 +600,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(P
     if (fill)
         strided_tile<L, P, IN_FILL32H, 2, RM, IG>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
     else
-        strided_tile<L, P, IN_LOAD, 1, RM, 0>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
+        strided_tile<LY, PY, IN_LOAD, 1, RMY, 0>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
 }
 
 // ------------------------------------------------------------ C2R pass along x

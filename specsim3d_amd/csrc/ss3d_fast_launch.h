@@ -28,12 +28,26 @@ hipError_t strided_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) 
                     nblocks, nbatch, a, s, attr);
 }
 
-template <int L, int RM, int IG>
+// y lengths of a fused launch beside a fill of length 2^L and plan RM: 2^L (plan RM) or, with
+// plan 3, 2^(L+1) where the tiles have one thread count and two blocks fit the CU's LDS
+template <int L, int RM>
+constexpr bool fused_up() {
+    if constexpr (L + 1 > kMaxFastLog) {
+        return false;
+    } else {
+        using CZ = fast::StridedCfg<L, strided_P(L), RM>;
+        using CY = fast::StridedCfg<L + 1, strided_P(L + 1), 3>;
+        return CZ::NT == CY::NT && 2 * (CZ::lds_bytes() > CY::lds_bytes() ? CZ::lds_bytes() : CY::lds_bytes()) <= 163840;
+    }
+}
+
+template <int L, int RM, int IG, int LY = L, int RMY = RM>
 hipError_t fused_L(int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     using C = fast::StridedCfg<L, strided_P(L), RM>;
+    using CY = fast::StridedCfg<LY, strided_P(LY), RMY>;
     static bool attr = false;
-    const auto k = fast::fill_load_kernel<L, strided_P(L), RM, IG>;
-    const size_t lds = C::lds_bytes();
+    const auto k = fast::fill_load_kernel<L, strided_P(L), RM, IG, LY, strided_P(LY), RMY>;
+    const size_t lds = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
     if (lds > 65536 && !attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -43,21 +57,38 @@ hipError_t fused_L(int nblocks, const PassArgs& az, const PassArgs& ay, const Fu
     return hipGetLastError();
 }
 
+template <int RM, int IG, int L>
+hipError_t fused_one(int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    if (LY == L) return fused_L<L, RM, IG>(nblocks, az, ay, m, s);
+    if constexpr (fused_up<L, RM>()) {
+        if (LY == L + 1) return fused_L<L, RM, IG, L + 1, 3>(nblocks, az, ay, m, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <int RM, int IG, int... Ls>
-hipError_t fused_dispatch(int L, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
+hipError_t fused_dispatch(int L, int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
                           hipStream_t s, std::integer_sequence<int, Ls...>) {
     hipError_t e = hipErrorInvalidValue;
-    ((L == Ls + 1 ? (e = fused_L<Ls + 1, RM, IG>(nblocks, az, ay, m, s), 0) : 0), ...);
+    ((L == Ls + 1 ? (e = fused_one<RM, IG, Ls + 1>(LY, nblocks, az, ay, m, s), 0) : 0), ...);
     return e;
 }
 
-// z fill (az) beside the y pass (ay), both of length 2^L, in nblocks = m.nzb + m.nyb blocks
+// whether a fill of length 2^L (plan RM) and a y pass of length 2^LY (plan rmy) have a fused launch
+template <int RM, int... Ls>
+bool fusable(int L, int LY, int rmy, std::integer_sequence<int, Ls...>) {
+    bool ok = false;
+    ((L == Ls + 1 ? (ok = (LY == L && rmy == RM) || (LY == L + 1 && rmy == 3 && fused_up<Ls + 1, RM>()), 0) : 0), ...);
+    return ok;
+}
+
+// z fill (az, length 2^L, plan RM) beside the y pass (ay, length 2^LY) in nblocks = m.nzb + m.nyb blocks
 template <int RM>
-hipError_t fused_rm(int L, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+hipError_t fused_rm(int L, int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     static_assert(2 * sizeof(PassArgs) + sizeof(FuseMap) <= 4096, "kernel arguments");
     using seq = std::make_integer_sequence<int, kMaxFastLog>;
-    return az.igauss ? fused_dispatch<RM, 1>(L, nblocks, az, ay, m, s, seq{})
-                     : fused_dispatch<RM, 0>(L, nblocks, az, ay, m, s, seq{});
+    return az.igauss ? fused_dispatch<RM, 1>(L, LY, nblocks, az, ay, m, s, seq{})
+                     : fused_dispatch<RM, 0>(L, LY, nblocks, az, ay, m, s, seq{});
 }
 
 template <int L, int IN, int RM>

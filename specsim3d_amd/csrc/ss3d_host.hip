@@ -540,7 +540,7 @@ int run_fused(ss3d_context* c, Pass& pz, int nbz, Pass& py, int nby, hipStream_t
         if (!e0 || !e1) return fail(SS3D_EHIP, "hipEventCreate failed");
         HIP_TRY(hipEventRecord(e0, strm));
     }
-    hipError_t e = launch_fast_fused(pz.a.logn, pz.rm, pz.a, py.a, m, strm);
+    hipError_t e = launch_fast_fused(pz.a.logn, pz.rm, py.a.logn, pz.a, py.a, m, strm);
     if (e != hipSuccess) return fail(SS3D_EHIP, "fused pass launch: %s", hipGetErrorString(e));
     if (c->profiling) {
         HIP_TRY(hipEventRecord(e1, strm));
@@ -1068,13 +1068,14 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
         if (has_z && has_y) SS3D_TRY(ensure2(L, L.wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
     }
     const int nbat = (nreal + nb - 1) / nb;
-    // fusion only where the y pass's own radix plan is the fill's (results do not depend on
-    // the schedule)
-    if (staged && has_z && has_y && !pipelined && c->fuse && lz == ly && nbat >= 2 &&
-        fast_same_plan(ly, py.rm, pz.rm)) {
+    // fusion keeps the y pass's own radix plan (results do not depend on the schedule): the fill's
+    // plan at the same length, plan 3 at twice the length
+    const int rmy_fused = lz == ly ? pz.rm : 3;
+    if (staged && has_z && has_y && !pipelined && c->fuse && nbat >= 2 && fast_same_plan(ly, py.rm, rmy_fused) &&
+        fast_fusable(lz, pz.rm, ly, rmy_fused)) {
         // launch k: fill of batch k beside the y pass of batch k-1 (pipe buffers alternate),
         // then the x pass of batch k-1
-        py.rm = pz.rm;  // the same plan, one template instance for both roles
+        py.rm = rmy_fused;  // the same plan, the template instance of the fused kernel
         for (int k = 0; k <= nbat; ++k) {
             const bool zk = k < nbat, yk = k >= 1;
             const int cz = zk ? std::min(nb, nreal - k * nb) : 0;
@@ -1259,7 +1260,8 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
     SS3D_TRY(ensure2(c->main, c->main.dout, (size_t)nb * ncrop * 4));
     SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)nb * 3 * sizeof(double)));
     const bool was = c->profiling;
-    double best[3] = {1e30, 1e30, 1e30}, tzy[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
+    double best[3] = {1e30, 1e30, 1e30}, tz[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
+    double ty[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
     int pick[3] = {5, 5, 5};
     for (int rm = 3; rm <= 5; ++rm) {
         for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = rm;
@@ -1275,15 +1277,27 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
                 best[k] = ms[6 + k];
                 pick[k] = rm;
             }
-        if (nl[6] && nl[7]) tzy[rm] = ms[6] + ms[7];
+        if (nl[6]) tz[rm] = ms[6];
+        if (nl[7]) ty[rm] = ms[7];
     }
     SS3D_TRY(ss3d_context_profile(c, was ? 1 : 0));
-    // z and y of one length run fused only with one radix plan: pick the plan of both by their sum
-    if (lz == ly && sp->npy > 1 && sp->npz > 1 && !fast_same_plan(lz, pick[0], pick[1])) {
-        int rm = 3;
-        for (int r = 4; r <= 5; ++r)
-            if (tzy[r] < tzy[rm]) rm = r;
-        pick[0] = pick[1] = rm;
+    // The fill and the y pass run fused only with plans that have a fused kernel (one plan at one
+    // length, plan 3 for a y pass twice as long): where one exists, take the pair with the
+    // smallest sum of their times.
+    if (sp->npy > 1 && sp->npz > 1 && !fast_fusable(lz, pick[0], ly, pick[1])) {
+        double bestsum = 1e30;
+        int pz = 0, py = 0;
+        for (int a = 3; a <= 5; ++a)
+            for (int b = 3; b <= 5; ++b)
+                if (fast_fusable(lz, a, ly, b) && tz[a] + ty[b] < bestsum) {
+                    bestsum = tz[a] + ty[b];
+                    pz = a;
+                    py = b;
+                }
+        if (pz) {
+            pick[0] = pz;
+            pick[1] = py;
+        }
     }
     for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = pick[k];
     return SS3D_OK;

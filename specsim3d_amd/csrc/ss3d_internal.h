@@ -110,7 +110,11 @@ struct FuseMap {
     int A, C, K;
 };
 bool fast_same_plan(int L, int rm1, int rm2);  // the FP32 radix plans of length 2^L for rm1 and rm2 are one
-hipError_t launch_fast_fused(int L, int rm, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s);
+// a fused launch exists for a fill of length 2^L (plan rm) and a y pass of length 2^LY (plan rmy):
+// LY = L with rmy = rm, or LY = L + 1 with rmy = 3 where the tiles have one thread count
+bool fast_fusable(int L, int rm, int LY, int rmy);
+hipError_t launch_fast_fused(int L, int rm, int LY, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
+                             hipStream_t s);
 
 // FP64 realization passes with compile-time lengths 2^1 .. 2^kMaxFast64Log (ss3d_fast64.h), bit
 // for bit the arithmetic of launch_pass64.  kind 4: x pass (fill + transform of rows, OUT_STORE),

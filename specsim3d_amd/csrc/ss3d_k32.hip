@@ -6,9 +6,9 @@ namespace ss3d {
 hipError_t launch_fast_strided_r3(int, int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_strided_r4(int, int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_strided_r5(int, int, int, int, int, const PassArgs&, hipStream_t);
-hipError_t launch_fast_fused_r3(int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
-hipError_t launch_fast_fused_r4(int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
-hipError_t launch_fast_fused_r5(int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
+hipError_t launch_fast_fused_r3(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
+hipError_t launch_fast_fused_r4(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
+hipError_t launch_fast_fused_r5(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
 hipError_t launch_fast_c2r_r3(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r4(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r5(int, int, int, int, const PassArgs&, hipStream_t);
@@ -53,12 +53,24 @@ bool fast_same_plan(int L, int rm1, int rm2) {
     return true;
 }
 
-hipError_t launch_fast_fused(int L, int rm, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+bool fast_fusable(int L, int rm, int LY, int rmy) {
+    using seq = std::make_integer_sequence<int, kMaxFastLog>;
+    if (L < 1 || L > kMaxFastLog) return false;
+    switch (rm) {
+        case 3: return fastlaunch::fusable<3>(L, LY, rmy, seq{});
+        case 4: return fastlaunch::fusable<4>(L, LY, rmy, seq{});
+        case 5: return fastlaunch::fusable<5>(L, LY, rmy, seq{});
+        default: return false;
+    }
+}
+
+hipError_t launch_fast_fused(int L, int rm, int LY, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
+                             hipStream_t s) {
     if (L < 1 || L > kMaxFastLog) return hipErrorInvalidValue;
     switch (rm) {
-        case 3: return launch_fast_fused_r3(L, az, ay, m, s);
-        case 4: return launch_fast_fused_r4(L, az, ay, m, s);
-        case 5: return launch_fast_fused_r5(L, az, ay, m, s);
+        case 3: return launch_fast_fused_r3(L, LY, az, ay, m, s);
+        case 4: return launch_fast_fused_r4(L, LY, az, ay, m, s);
+        case 5: return launch_fast_fused_r5(L, LY, az, ay, m, s);
         default: return hipErrorInvalidValue;
     }
 }

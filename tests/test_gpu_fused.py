@@ -121,3 +121,43 @@ def test_autotune_keeps_fusion(eng):
         sp.close()
     say(f"plan {plan}")
     assert "real_zy_fp32" in kinds, (plan, kinds)
+
+
+# fill and y pass of different lengths (y twice as long, y plan 3): (structures, grid, realizations,
+# fill plan) -- periodic 128 x 256 x 128 and 64 x 1024 x 512
+CASES_UP = [
+    ([(1, 1.0, 20.0, 20.0, 20.0, 0.0, 0.0, 0.0)], (64, 200, 64), 19, 3),
+    ([(2, 1.0, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0)], (40, 600, 300), 10, 4),
+]
+
+
+@pytest.mark.parametrize("sts,grid,nreal,rmz", CASES_UP)
+def test_fused_longer_y_pass(eng, sts, grid, nreal, rmz):
+    """A y pass twice as long as the fill (plan 3, tiles of the fill's thread count) runs fused
+    and gives the fields of separate launches bit for bit."""
+    nx, ny, nz = grid
+    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    lz, ly = int(np.log2(npg[2])), int(np.log2(npg[1]))
+    assert ly == lz + 1
+    say(f"[fused] fill 2^{lz} plan {rmz} beside y 2^{ly} plan 3, grid {grid}")
+    old = {6: eng.tuning_get(6, lz), 7: eng.tuning_get(7, ly)}
+    sp = eng.spectrum(sts, 0.0, 1.0, 1.0, 1.0, *npg)
+    try:
+        eng.tuning_set(6, lz, rmz)
+        eng.tuning_set(7, ly, 3)
+        got = {}
+        for opt in (1, 0):
+            eng.set_option(S.OPT_FUSE, opt)
+            eng.profile(True)
+            got[opt] = eng.realize(sp, 99, 1, 2, nreal, nx, ny, nz, precision=S.FP32)
+            eng.profile(False)
+            kinds = eng.profile_read()
+            assert ("real_zy_fp32" in kinds) == (opt == 1), kinds
+    finally:
+        eng.set_option(S.OPT_FUSE, 1)
+        eng.tuning_set(6, lz, old[6])
+        eng.tuning_set(7, ly, old[7])
+        sp.close()
+    (f1, m1, v1, _), (f0, m0, v0, _) = got[1], got[0]
+    assert np.array_equal(f1, f0)
+    assert np.array_equal(m1, m0) and np.array_equal(v1, v0)
