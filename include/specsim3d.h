@@ -150,9 +150,10 @@ int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
 /* SS3D_OPT_FP64_GENERIC (default 0): FP64 realizations through the generic (runtime-length) pass
  * kernel instead of the compile-time kernels; both compute the same values bit for bit. */
 #define SS3D_OPT_FP64_GENERIC 2
-/* SS3D_OPT_FUSE (default 1): FP32 realizations with z and y axes of one length run the fill pass
- * of batch k+1 and the y pass of batch k in one launch whose blocks interleave (arithmetic-bound
- * and memory-bound tiles share each CU); 0 runs them as separate launches.  Same values. */
+/* SS3D_OPT_FUSE (default 1): FP32 realizations whose z and y axes have one length (or a y axis
+ * twice as long) run the fill pass of batch k+1 and the y pass of batch k in one launch whose
+ * blocks interleave (arithmetic-bound and memory-bound tiles share each CU), where the passes' radix
+ * plans have such a launch; 0 runs them as separate launches.  Same values. */
 #define SS3D_OPT_FUSE 3
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
