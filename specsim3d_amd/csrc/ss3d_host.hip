@@ -1074,26 +1074,14 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     if (staged && has_z && has_y && !pipelined && c->fuse && nbat >= 2 && fast_same_plan(ly, py.rm, rmy_fused) &&
         fast_fusable(lz, pz.rm, ly, rmy_fused)) {
         // launch k: fill of batch k beside the y pass of batch k-1 (pipe buffers alternate),
-        // then the x pass of batch k-1.  The first and last batches hold half a batch where the
-        // count allows, so the unfused first fill and last y pass are short.
+        // then the x pass of batch k-1
         py.rm = rmy_fused;  // the same plan, the template instance of the fused kernel
-        std::vector<int> b0{0};  // batch starts, then nreal
-        const int half = nb / 2;
-        if (half >= 1 && nreal >= 2 * nb) {
-            b0.push_back(half);
-            while (nreal - b0.back() > nb + half) b0.push_back(b0.back() + nb);
-            if (nreal - b0.back() > half) b0.push_back(nreal - half);
-        } else {
-            while (nreal - b0.back() > nb) b0.push_back(b0.back() + nb);
-        }
-        b0.push_back(nreal);
-        const int nb2 = (int)b0.size() - 1;  // batches
-        for (int k = 0; k <= nb2; ++k) {
-            const bool zk = k < nb2, yk = k >= 1;
-            const int cz = zk ? b0[k + 1] - b0[k] : 0;
-            const int p0 = yk ? b0[k - 1] : 0, cy = yk ? b0[k] - b0[k - 1] : 0;
+        for (int k = 0; k <= nbat; ++k) {
+            const bool zk = k < nbat, yk = k >= 1;
+            const int cz = zk ? std::min(nb, nreal - k * nb) : 0;
+            const int p0 = (k - 1) * nb, cy = yk ? std::min(nb, nreal - p0) : 0;
             if (zk) {
-                pz.a.rbase = L.rbase.as<uint32_t>() + (size_t)b0[k];
+                pz.a.rbase = L.rbase.as<uint32_t>() + (size_t)k * nb;
                 pz.a.out = L.pipe[k & 1].p;
             }
             if (yk) {
