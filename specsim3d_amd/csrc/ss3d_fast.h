@@ -330,7 +330,10 @@ __device__ __forceinline__ void st_nt16(float4 v, float4* p) {
 // computed once and kept in registers; per realization only its base factor is multiplied in
 // (the Park-Miller state is a product mod 2^31-1, so the order of the factors does not matter).
 // One tile of a strided pass (the work of one block): `tile` in XCD order, member group `grp`.
-template <int L, int P, int IN, int AXIS, int RM, int IG>
+// HALF (fill passes that keep at most half the axis, n_out <= N/2): the last radix pass computes
+// only the outputs r < R/2 of each butterfly.  Output r of a last-pass butterfly sits at position
+// pos0 + r N/R with pos0 < N/R, so the outputs r >= R/2 lie at N/2 or beyond and are never stored.
+template <int L, int P, int IN, int AXIS, int RM, int IG, bool HALF = false>
 __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, const int grp) {
     using C = StridedCfg<L, P, RM>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, PH = P / 2;
@@ -425,7 +428,8 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
 #pragma unroll
                     for (int r = 0; r < (1 << LRl); ++r) {
                         const int e = i * (1 << LRl) + r;
-                        if (wpos0 + (r << NSLl) < nout) bstore16(pack2(v0[e], v1[e]), rs, o0 + r * step * 16u);
+                        if ((!HALF || r < (1 << LRl) / 2) && wpos0 + (r << NSLl) < nout)
+                            bstore16(pack2(v0[e], v1[e]), rs, o0 + r * step * 16u);
                     }
                 }
             } else if (active) {
@@ -437,7 +441,8 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
 #pragma unroll
                     for (int r = 0; r < (1 << LRl); ++r) {
                         const int e = i * (1 << LRl) + r;
-                        if (pos0 + (r << NSLl) < nout) st_nt16(pack2(v0[e], v1[e]), d0 + (int64_t)r * step);
+                        if ((!HALF || r < (1 << LRl) / 2) && pos0 + (r << NSLl) < nout)
+                            st_nt16(pack2(v0[e], v1[e]), d0 + (int64_t)r * step);
                     }
                 }
             }
@@ -582,7 +587,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(Pas
 // CU holds arithmetic-bound fill tiles and memory-bound y tiles at once.  Role-local block l runs
 // tile l mod nblocks (in XCD order) of member group l / nblocks of its pass; 8A and 8C keep l and
 // the block index equal mod 8 (the same XCD).
-template <int L, int P, int RM, int IG, int LY = L, int PY = P, int RMY = RM>
+template <int L, int P, int RM, int IG, int LY = L, int PY = P, int RMY = RM, bool HALF = false>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(PassArgs az, PassArgs ay, FuseMap m) {
     static_assert(StridedCfg<LY, PY, RMY>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
     const int per = 8 * (m.A + m.C), bid = blockIdx.x, full = m.K * per;
@@ -598,7 +603,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(P
         l = fill ? 8 * m.A * m.K + o : 8 * m.C * m.K + (o - zr);
     }
     if (fill)
-        strided_tile<L, P, IN_FILL32H, 2, RM, IG>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
+        strided_tile<L, P, IN_FILL32H, 2, RM, IG, HALF>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
     else
         strided_tile<LY, PY, IN_LOAD, 1, RMY, 0>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
 }

@@ -41,12 +41,12 @@ constexpr bool fused_up() {
     }
 }
 
-template <int L, int RM, int IG, int LY = L, int RMY = RM>
+template <int L, int RM, int IG, int LY = L, int RMY = RM, bool HALF = false>
 hipError_t fused_L(int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     using C = fast::StridedCfg<L, strided_P(L), RM>;
     using CY = fast::StridedCfg<LY, strided_P(LY), RMY>;
     static bool attr = false;
-    const auto k = fast::fill_load_kernel<L, strided_P(L), RM, IG, LY, strided_P(LY), RMY>;
+    const auto k = fast::fill_load_kernel<L, strided_P(L), RM, IG, LY, strided_P(LY), RMY, HALF>;
     const size_t lds = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
     if (lds > 65536 && !attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -57,8 +57,20 @@ hipError_t fused_L(int nblocks, const PassArgs& az, const PassArgs& ay, const Fu
     return hipGetLastError();
 }
 
+// fills of length >= 2^kHalfMinLog that keep at most half the axis use the HALF instance
+constexpr int kHalfMinLog = 7;
+
 template <int RM, int IG, int L>
 hipError_t fused_one(int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    if constexpr (L >= kHalfMinLog) {
+        if (2 * az.n_out <= (1 << L)) {
+            if (LY == L) return fused_L<L, RM, IG, L, RM, true>(nblocks, az, ay, m, s);
+            if constexpr (fused_up<L, RM>()) {
+                if (LY == L + 1) return fused_L<L, RM, IG, L + 1, 3, true>(nblocks, az, ay, m, s);
+            }
+            return hipErrorInvalidValue;
+        }
+    }
     if (LY == L) return fused_L<L, RM, IG>(nblocks, az, ay, m, s);
     if constexpr (fused_up<L, RM>()) {
         if (LY == L + 1) return fused_L<L, RM, IG, L + 1, 3>(nblocks, az, ay, m, s);
