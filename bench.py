@@ -469,6 +469,9 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-writer", action="store_true")
     ap.add_argument("--no-cond", action="store_true")
+    ap.add_argument("--plan", default="",
+                    help="radix plans Z,Y,X (log2 of the largest radix of the FP32 z fill, y and C2R x passes) "
+                         "instead of autotuning, so that profiling runs launch the same kernels")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
@@ -497,7 +500,14 @@ def main():
     nreal = args.nreal
     ncell = nx * ny * nz
     t0 = time.perf_counter()
-    plan = eng.autotune(spec, nx, ny, nz, IGAUSS, reps=3)
+    if args.plan:
+        lx, ly, lz = (int(v).bit_length() - 1 for v in grid)
+        kinds = {"real_z_fp32": (6, lz), "real_y_fp32": (7, ly), "real_x_c2r_fp32": (8, lx - 1)}
+        for (kind, logn), rm in zip(kinds.values(), (int(v) for v in args.plan.split(","))):
+            eng.tuning_set(kind, logn, rm)
+        plan = {k: eng.tuning_get(*v) for k, v in kinds.items()}
+    else:
+        plan = eng.autotune(spec, nx, ny, nz, IGAUSS, reps=3)
     tune_s = time.perf_counter() - t0
     out32 = hip.malloc(nreal * ncell * 4)
     stats = hip.malloc(nreal * 3 * 8)
