@@ -330,7 +330,7 @@ __device__ __forceinline__ void st_nt16(float4 v, float4* p) {
 // computed once and kept in registers; per realization only its base factor is multiplied in
 // (the Park-Miller state is a product mod 2^31-1, so the order of the factors does not matter).
 // One tile of a strided pass (the work of one block): `tile` in XCD order, member group `grp`.
-// HALF (fill passes that keep at most half the axis, n_out <= N/2): the last radix pass computes
+// HALF (passes that keep at most half the axis, n_out <= N/2): the last radix pass computes
 // only the outputs r < R/2 of each butterfly.  Output r of a last-pass butterfly sits at position
 // pos0 + r N/R with pos0 < N/R, so the outputs r >= R/2 lie at N/2 or beyond and are never stored.
 template <int L, int P, int IN, int AXIS, int RM, int IG, bool HALF = false>
@@ -408,7 +408,7 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                     const int pos = out_pos<L, M - 1, RM>(g + G * i, r);
                     const int e = i * (1 << LRl) + r;
                     const uint32_t off = (active && pos < nout) ? (q + (uint32_t)pos * sA) * 16u : kOob;
-                    bstore16(pack2(v0[e], v1[e]), rs, off);
+                    if (!HALF || r < (1 << LRl) / 2) bstore16(pack2(v0[e], v1[e]), rs, off);
                 }
         } else {
             // fill passes: element r of butterfly i sits at position pos(i, 0) + r 2^NSL.  When the
@@ -478,7 +478,8 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
             // prefetched operands does not also wait for the previous member's stores
             const __amdgpu_buffer_rsrc_t rs = rsrc(a.out);
 #pragma unroll
-            for (int e = 0; e < EPT; ++e) bstore16(make_float4(0.f, 0.f, 0.f, 0.f), rs, kOob + 16u * e);
+            for (int e = 0; e < (HALF ? EPT / 2 : EPT); ++e)
+                bstore16(make_float4(0.f, 0.f, 0.f, 0.f), rs, kOob + 16u * e);
         }
         consume();
         for (int b = b0; b < b1; ++b) {
@@ -605,7 +606,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(P
     if (fill)
         strided_tile<L, P, IN_FILL32H, 2, RM, IG, HALF>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
     else
-        strided_tile<LY, PY, IN_LOAD, 1, RMY, 0>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
+        strided_tile<LY, PY, IN_LOAD, 1, RMY, 0, HALF>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
 }
 
 // ------------------------------------------------------------ C2R pass along x

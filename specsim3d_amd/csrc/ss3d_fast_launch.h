@@ -57,13 +57,14 @@ hipError_t fused_L(int nblocks, const PassArgs& az, const PassArgs& ay, const Fu
     return hipGetLastError();
 }
 
-// fills of length >= 2^kHalfMinLog that keep at most half the axis use the HALF instance
+// fills of length >= 2^kHalfMinLog whose z and y passes both keep at most half their axes use
+// the HALF instance
 constexpr int kHalfMinLog = 7;
 
 template <int RM, int IG, int L>
 hipError_t fused_one(int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     if constexpr (L >= kHalfMinLog) {
-        if (2 * az.n_out <= (1 << L)) {
+        if (2 * az.n_out <= (1 << L) && 2 * (int64_t)ay.n_out <= (int64_t(1) << LY)) {
             if (LY == L) return fused_L<L, RM, IG, L, RM, true>(nblocks, az, ay, m, s);
             if constexpr (fused_up<L, RM>()) {
                 if (LY == L + 1) return fused_L<L, RM, IG, L + 1, 3, true>(nblocks, az, ay, m, s);

@@ -128,6 +128,8 @@ def test_autotune_keeps_fusion(eng):
 CASES_UP = [
     ([(1, 1.0, 20.0, 20.0, 20.0, 0.0, 0.0, 0.0)], (64, 200, 64), 19, 3),
     ([(2, 1.0, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0)], (40, 600, 300), 10, 4),
+    # both passes keep at most half their axes (z 60 of 128, y 120 of 256): the pruned instance
+    ([(2, 1.0, 10.0, 40.0, 10.0, 0.0, 0.0, 0.0)], (48, 120, 60), 19, 3),
 ]
 
 
