@@ -23,6 +23,8 @@ CASES = [
     ([(3, 1.0, 6.0, 6.0, 6.0, 0.0, 0.0, 0.0)], 0.0, (60, 30, 30), 40, 0, True),
     ([(2, 1.0, 3.0, 3.0, 3.0, 0.0, 0.0, 0.0)], 0.0, (16, 16, 16), 600, 1, True),
     ([(1, 1.0, 30.0, 30.0, 30.0, 0.0, 0.0, 0.0)], 0.0, (100, 100, 20), 17, 1, False),
+    # 128x256x256: z 90 and y 120 of 256, the pruned instance at length 2^8, igauss=0
+    ([(1, 0.9, 40.0, 40.0, 40.0, 0.0, 0.0, 0.0)], 0.1, (50, 120, 90), 20, 0, True),
 ]
 
 
