@@ -2,9 +2,11 @@
 pass of batch k run in one launch whose blocks interleave.  The launch performs the arithmetic
 of the separate passes on the same tiles, so the fields and statistics equal those of separate
 launches bit for bit.  The cases exercise the block-role map: more fill blocks than y blocks and
-fewer (a short last batch), a tail of one role, lengths 2^5 .. 2^7, both coefficient laws, and a
-grid whose z and y lengths differ (no fusion: separate launches either way).  The fused pass is
-what runs: pass kind 9 appears in the profile only with the option on.
+fewer (a short last batch), a tail of one role, lengths 2^5 .. 2^10, both coefficient laws, a y
+axis twice the fill's, the instance that prunes the upper half of the last radix pass (both passes
+keep at most half their axes) and the one that does not, and a grid whose z and y lengths differ
+otherwise (no fusion: separate launches either way).  The fused pass is what runs: pass kind 9
+appears in the profile only with the option on.
 """
 import sys
 
