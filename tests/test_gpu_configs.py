@@ -1,0 +1,167 @@
+"""The benchmark workloads at full size, pinned to the reference where it runs.
+
+256^3 grid in a 512^3 periodic grid (exponential sill 0.7 + nugget 0.3, ranges 80/40/10, az 30,
+dip 15, igauss=1, seed 69069): the FP64 drop-in path against the reference program's own FP64
+realizations 1 and 2 (tests/golden/c2, written by tests/golden/make_c2_golden.py from the reference
+built by oracle/Makefile): 32768 sampled values, a whole x row and z column per realization, the
+sums over the whole field and the printed statistics.  Bars: 1e-10 relative (max-norm) for the
+FP64 path, 1e-5 relative RMS for the FP32 path.
+
+512x512x256 grid in a 1024x1024x512 periodic grid (gaussian sill 1, ranges 150/60/20, az 120,
+dip -20, igauss=1): 2^29 periodic nodes, above the 2^27 the reference accepts
+(src/specsim3d.f90:1253), so the FP64 drop-in path is the oracle here; that path is pinned to the
+reference below 2^27 by the golden cases.  FP32 vs FP64 on the same draws <= 1e-5 relative RMS for
+the first and last of 32 realizations; the device variograms along x, y and z equal a NumPy
+shifted-difference on the same field; the ensemble variogram follows the model.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import specsim3d_amd as S
+from hipmem import DevBuf
+
+pytestmark = pytest.mark.gpu
+
+C2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = S.Engine(0)
+    yield e
+    e.close()
+
+
+def rel_max(a, ref):
+    return float(np.abs(a - ref).max() / np.abs(ref).max())
+
+
+def rel_rms(a, ref):
+    return float(np.sqrt(np.mean((a - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+
+
+def c2_fixture():
+    with open(os.path.join(C2, "c2.json")) as fh:
+        meta = json.load(fh)
+    with np.load(os.path.join(C2, "c2_sample.npz"), allow_pickle=False) as z:
+        return meta, {k: z[k] for k in z.files}
+
+
+def printed_stats(meta):
+    out = []
+    for ln in meta["stdout"]:
+        if ln.strip().startswith("realization"):
+            p = ln.replace("=", " ").split()
+            out.append((float(p[3]), float(p[5])))
+    return out
+
+
+def check_against_fixture(f, fx, tol, rel):
+    nreal = fx["val"].shape[0]
+    flat = f.reshape(nreal, -1)
+    for r in range(nreal):
+        assert rel(flat[r, fx["idx"]].astype(np.float64), fx["val"][r]) < tol, r
+        assert rel(f[r, 0, 0, :].astype(np.float64), fx["rowx"][r]) < tol, r
+        assert rel(f[r, :, 200, 17].astype(np.float64), fx["colz"][r]) < tol, r
+
+
+def test_c2_fp64_and_fp32_match_reference(eng):
+    meta, fx = c2_fixture()
+    sts = [tuple(s) for s in meta["structures"]]
+    nx, ny, nz = meta["grid"]
+    grid = S.periodic_grid(sts, *meta["spacing"], nx, ny, nz, meta["ipad"])
+    assert list(grid) == meta["periodic_grid"] == [512, 512, 512]
+    say("[c2] spectrum 512^3")
+    sp = eng.spectrum(sts, meta["nugget"], *meta["spacing"], *grid)
+    try:
+        d = sp.diag
+        # printed diagnostics: max|Im S|/max|Re S| (es10.3), C(0) and the positive mass (f11.5)
+        pr = {ln.split(":")[0].strip(): ln.split(":")[1].strip() for ln in meta["stdout"] if ":" in ln}
+        assert f"{d['max_imag'] / d['max_real']:.3E}" == pr["max |Im S| / max |Re S|"]
+        assert f"{d['c0']:.5f}" == pr["target variance C(0)"]
+        assert f"{d['spos']:.5f}" == pr["sum of positive spectrum"]
+        say("[c2] fp64 realizations 1-2")
+        f64, m64, v64, mi64 = eng.realize(sp, meta["iseed"], meta["igauss"], 0, meta["nreal"], nx, ny, nz, S.FP64)
+        check_against_fixture(f64, fx, 1e-10, rel_max)
+        n = nx * ny * nz
+        flat = f64.reshape(meta["nreal"], n)
+        sums = np.stack([flat.sum(axis=1), (flat * flat).sum(axis=1)], axis=1)
+        assert np.allclose(sums[:, 0], fx["sums"][:, 0], rtol=0, atol=1e-10 * np.sqrt(fx["sums"][:, 1].max()))
+        assert np.allclose(sums[:, 1], fx["sums"][:, 1], rtol=1e-10, atol=0)
+        assert np.allclose(m64, fx["sums"][:, 0] / n, rtol=0, atol=1e-12)
+        for (m, v), mo, vo in zip(printed_stats(meta), m64, v64):
+            assert abs(m - mo) <= 6e-5 and abs(v - vo) <= 6e-5
+        assert (mi64 < 1e-10).all()
+        del f64, flat
+        say("[c2] fp32 realizations 1-2")
+        f32, m32, v32, _ = eng.realize(sp, meta["iseed"], meta["igauss"], 0, meta["nreal"], nx, ny, nz, S.FP32)
+        check_against_fixture(f32, fx, 1e-5, rel_rms)
+        assert np.allclose(m32, m64, rtol=0, atol=1e-5) and np.allclose(v32, v64, rtol=1e-5)
+    finally:
+        sp.close()
+
+
+def gamma_numpy(f, d, lag):
+    """Semivariogram of one realization f[z, y, x] at lag `lag` along axis direction d."""
+    ax = {0: 2, 1: 1, 2: 0}[d]
+    n = f.shape[ax]
+    a = np.take(f, np.arange(0, n - lag), axis=ax).astype(np.float64)
+    b = np.take(f, np.arange(lag, n), axis=ax).astype(np.float64)
+    return 0.5 * float(np.mean((a - b) ** 2))
+
+
+def test_c5_large_job(eng):
+    """512x512x256 in 1024x1024x512: 32 FP32 realizations in HBM; FP32 vs FP64 on realizations 0 and 31."""
+    sts = [(3, 1.0, 150.0, 60.0, 20.0, 120.0, -20.0, 0.0)]
+    nx, ny, nz, nreal = 512, 512, 256, 32
+    grid = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    assert grid == (1024, 1024, 512) and grid[0] * grid[1] * grid[2] > 1 << 27
+    ncell = nx * ny * nz
+    say("[c5] spectrum 1024x1024x512")
+    sp = eng.spectrum(sts, 0.0, 1.0, 1.0, 1.0, *grid)
+    out = DevBuf(nreal * ncell * 4)
+    stats = DevBuf(nreal * 3 * 8)
+    try:
+        d = sp.diag
+        assert d["c0"] == pytest.approx(1.0) and d["spos"] + d["sneg"] == pytest.approx(1.0, abs=1e-9)
+        say("[c5] 32 fp32 realizations")
+        eng.realize_device(sp, 4711, 1, 0, nreal, nx, ny, nz, S.FP32, out.ptr, stats.ptr)
+        eng.synchronize()
+        st = stats.get(np.float64, nreal * 3).reshape(nreal, 3)
+        dirs = [(1, 0, 0, 64), (0, 1, 0, 32), (0, 0, 1, 16)]
+        say("[c5] device variograms")
+        gam = eng.variogram(out.ptr, dirs, device_shape=(nreal, nz, ny, nx), precision=S.FP32)
+        for r in (0, nreal - 1):
+            say(f"[c5] fp64 realization {r}")
+            f64, m64, v64, _ = eng.realize(sp, 4711, 1, r, 1, nx, ny, nz, S.FP64)
+            f32 = out.get(np.float32, ncell, r * ncell).reshape(nz, ny, nx)
+            assert rel_rms(f32.astype(np.float64), f64[0]) < 1e-5
+            assert abs(st[r, 0] - m64[0]) < 1e-5 and abs(st[r, 1] - v64[0]) < 1e-5 * max(1.0, v64[0])
+            for k, (sx, sy, sz, nl) in enumerate(dirs):
+                dax = 0 if sx else (1 if sy else 2)
+                for lag in (1, nl // 2, nl):
+                    g = gamma_numpy(f32, dax, lag)
+                    assert gam[r, k, lag - 1] == pytest.approx(g, rel=1e-6), (r, k, lag)
+            del f64, f32
+    finally:
+        out.free()
+        stats.free()
+        sp.close()
+    # sample variance near the sill, every realization distinct
+    assert 0.5 < float(np.mean(st[:, 1])) < 1.5
+    # ensemble variogram against the model: along x (range 150 after rotation) the field is smooth,
+    # so the relative Monte-Carlo error of a 32-member mean is a few percent
+    g_mean = gam.mean(axis=0)
+    for k, (sx, sy, sz, nl) in enumerate(dirs):
+        for lag in (1, nl // 2, nl):
+            model = 1.0 - S.covariance(sts, 0.0, lag * sx, lag * sy, lag * sz)
+            assert abs(g_mean[k, lag - 1] - model) <= 0.25 * model + 1e-3, (k, lag, g_mean[k, lag - 1], model)
