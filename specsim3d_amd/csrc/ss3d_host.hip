@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/specsim3d.h"
+#include "ss3d_exp.h"
 #include "ss3d_internal.h"
 
 using namespace ss3d;

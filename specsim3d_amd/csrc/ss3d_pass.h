@@ -24,6 +24,7 @@
 // without contraction reproduces the reference transform bit for bit.
 #pragma once
 
+#include "ss3d_exp.h"
 #include "ss3d_internal.h"
 
 namespace ss3d {
@@ -141,7 +142,8 @@ __device__ __forceinline__ cpx<double> coef64(double amp, uint32_t st, int igaus
     return coef_of_draw(amp, draw64(st, igauss), igauss, d.conj);
 }
 
-// Covariance at a lag, f90:153-178, in the reference's operation order.
+// Covariance at a lag, f90:153-178, in the reference's operation order, with the exp of the
+// reference build's libm (ss3d_exp.h), so that the covariance grid is the reference's bit for bit.
 __device__ __forceinline__ double cov64(const PassArgs& a, double hx, double hy, double hz) {
     double c = 0.0;
     if (hx == 0.0 && hy == 0.0 && hz == 0.0) c = a.nugget;
@@ -155,9 +157,9 @@ __device__ __forceinline__ double cov64(const PassArgs& a, double hx, double hy,
         if (st.itype == 1) {
             if (hr < 1.0) c = c + st.sill * (1.0 - 1.5 * hr + 0.5 * (hr * hr * hr));
         } else if (st.itype == 2) {
-            c = c + st.sill * exp(-3.0 * hr);
+            c = c + st.sill * libm::exp(-3.0 * hr);
         } else {
-            c = c + st.sill * exp(-3.0 * hr * hr);
+            c = c + st.sill * libm::exp(-3.0 * hr * hr);
         }
     }
     return c;
