@@ -124,6 +124,26 @@ int ss3d_realize(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int ig
 int ss3d_realize_device(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int igauss, int64_t r0,
                         int nreal, int nx, int ny, int nz, int precision, void* dev_out, double* dev_stats);
 
+/* The same work as ss3d_realize_device, with no device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc, hipFree or other allocating routine.
+ * Every work buffer of its passes is a window of one static device array of SS3D_SCRATCH_BYTES
+ * (256 MiB) that each GPU holds from the library's first use on it until the process ends; the
+ * caller still passes the output (and optionally the statistics) buffer.  Results and statistics
+ * are those of ss3d_realize_device bit for bit.
+ *  - SS3D_ETOOBIG when one realization of the grid needs more scratch than the array holds (the
+ *    message gives both sizes); e.g. 128^3 outputs in a 256^3 periodic grid fit (FP32: 3
+ *    realizations per launch, FP64: 1), 256^3 in 512^3 does not.
+ *  - SS3D_EINVAL when the spectrum's random-stream tables for igauss are not built yet: any one
+ *    ss3d_realize / ss3d_realize_device call with that igauss builds them.
+ *  - Every call on a GPU uses that GPU's one array (which keeps the last call's contents; no
+ *    buffer needs clearing, every word read is written earlier in the same call).  Calls ordered
+ *    on one context's stream run one after the other; calls that overlap on one GPU (two contexts,
+ *    or two host threads) race.  Ordered on the context stream; no profiling records. */
+#define SS3D_SCRATCH_BYTES (256ll << 20)
+int ss3d_realize_device_noalloc(ss3d_context* ctx, const ss3d_spectrum* spec, int iseed, int igauss, int64_t r0,
+                                int nreal, int nx, int ny, int nz, int precision, void* dev_out, double* dev_stats);
+
 /* Per-pass timing with HIP events recorded on the context stream around every pass launch.
  * Pass kinds: 0..2 spectrum x/y/z (FP64), 3..5 FP64 realization x/y/z, 6..8 FP32 realization
  * z/y/x, 9 FP32 z fill beside the y pass of the previous batch (SS3D_OPT_FUSE).  enable=1 clears and starts recording, enable=0 stops.  profile_read synchronises the

@@ -23,6 +23,7 @@ FP32 = 1
 OPT_PIPELINE = 1
 OPT_FP64_GENERIC = 2
 OPT_FUSE = 3
+SCRATCH_BYTES = 256 << 20  # SS3D_SCRATCH_BYTES: the scratch array of ss3d_realize_device_noalloc
 
 ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device", 6: "I/O error"}
 
@@ -121,6 +122,8 @@ SIGNATURES = {
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _dp, _dp, _dp]),
     "ss3d_realize_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "ss3d_realize_device_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "ss3d_context_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ss3d_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ss3d_tuning_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
@@ -321,6 +324,12 @@ class Engine:
         """Asynchronous variant: dev_out / dev_stats are device addresses (ints)."""
         check(lib().ss3d_realize_device(self.handle, spec.handle, iseed, igauss, r0, nreal, nx, ny, nz, precision,
                                         _vp(dev_out), _vp(dev_stats) if dev_stats else None))
+
+    def realize_device_noalloc(self, spec, iseed, igauss, r0, nreal, nx, ny, nz, precision, dev_out, dev_stats=None):
+        """realize_device with no device allocation: work buffers in the library's fixed scratch
+        array (SCRATCH_BYTES per GPU); grids whose one realization needs more raise SS3DError."""
+        check(lib().ss3d_realize_device_noalloc(self.handle, spec.handle, iseed, igauss, r0, nreal, nx, ny, nz,
+                                                precision, _vp(dev_out), _vp(dev_stats) if dev_stats else None))
 
     def synchronize(self):
         check(lib().ss3d_context_synchronize(self.handle))

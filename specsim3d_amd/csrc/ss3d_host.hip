@@ -29,6 +29,14 @@
 
 using namespace ss3d;
 
+// Scratch memory of ss3d_realize_device_noalloc: every work buffer of its passes (fill output,
+// y output, realization bases, block statistics) is a window of this array, so that the call
+// allocates nothing.  Each GPU holds its own copy from the first use of the library on it until
+// the process ends.  In host code `scratch` only names the symbol: its device address comes from
+// hipGetSymbolAddress (scratch_address below), taken once per device.
+constexpr size_t kScratchBytes = size_t(SS3D_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 // ---------------------------------------------------------------- errors
 namespace {
 
@@ -145,14 +153,23 @@ int64_t draws_per_realization(int npx, int npy, int npz, int igauss) {
 }
 
 // ---------------------------------------------------------------- device buffers
+// A device allocation owned by the library, or (view) a window of `cap` bytes of memory owned
+// elsewhere -- the scratch array of ss3d_realize_device_noalloc -- which never grows or frees.
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool view = false;
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && !view) (void)hipFree(p);
+    }
+    void set_view(void* base, size_t bytes) {
+        p = base;
+        cap = bytes;
+        view = true;
     }
     int ensure(size_t bytes, hipStream_t s) {
         if (bytes <= cap) return SS3D_OK;
+        if (view) return fail(SS3D_ETOOBIG, "scratch window of %zu bytes cannot hold %zu", cap, bytes);
         if (p) {
             HIP_TRY(hipStreamSynchronize(s));
             HIP_TRY(hipFree(p));
@@ -230,6 +247,10 @@ struct ss3d_context {
     // FP32 W_n tables for n = 2^0..2^kMaxLogN: entry m = exp(+2 pi i m/n) at offset (n - 1) + m
     DevBuf tw32n;
     Lane main;               // the context's own lane (main.stream is `stream`)
+    // ss3d_realize_device_noalloc: the device address of `scratch` on this context's device, and
+    // a lane on the context's streams whose buffers are windows of it
+    void* scratch = nullptr;
+    Lane sl;
     bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
     bool fuse = true;        // SS3D_OPT_FUSE: FP32 fill of batch k+1 and y pass of batch k in one launch
     bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
@@ -570,6 +591,25 @@ int check_grid(int npx, int npy, int npz) {
 }  // namespace
 
 // ================================================================ C ABI
+namespace {
+// Device address of `scratch` on `device` (current), looked up once per device: each GPU holds
+// its own copy of the array at its own address.
+int scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(SS3D_EHIP, "hipGetSymbolAddress returned a null scratch address");
+        it = addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return SS3D_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int ss3d_version(void) { return SS3D_VERSION; }
@@ -591,6 +631,9 @@ int ss3d_context_create(int device, ss3d_context** out) {
     c->device = device;
     SS3D_TRY(c->main.init());
     c->stream = c->main.stream;
+    c->sl.stream = c->main.stream;
+    c->sl.stream2 = c->main.stream2;
+    SS3D_TRY(scratch_address(device, &c->scratch));
     SS3D_TRY(build_twiddles(c.get()));
     *out = c.release();
     return SS3D_OK;
@@ -975,8 +1018,35 @@ int ensure2(Lane& L, DevBuf& b, size_t bytes) {
 // first one) runs on c->stream and writes one of two pipe buffers; the other passes and the
 // statistics run on L.stream2, so the fill of batch k+1 (arithmetic-bound) overlaps the
 // memory-bound passes of batch k.
+// Work-buffer sizes of the FP32 passes per batch member, in complex elements: the fill pass's
+// output (column-tiled when the z and y axes have one length) and the y pass's output.
+struct Fp32Work {
+    int64_t fill, yout;
+    bool staged, has_wb;
+};
+Fp32Work fp32_work(const ss3d_spectrum* sp, int nx, int ny, int nz) {
+    const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
+    const bool has_y = npy > 1, has_z = npz > 1;
+    const int hc = npx >= 2 ? npx / 2 : 1, hcs = std::max(2, hc);
+    const int64_t bz = (int64_t)hcs * npy * nz, by = (int64_t)hcs * ny * nz;
+    int64_t bzs = bz;
+    if (has_z && has_y && ilog2(npz) == ilog2(npy)) {
+        const int P = fast_strided_tile(ilog2(npz));
+        const int64_t slab = (int64_t)npy * P, plane = (int64_t)((hc + P - 1) / P) * slab;
+        bzs = plane * nz;
+    }
+    Fp32Work w;
+    w.staged = has_y || has_z;
+    w.has_wb = has_z && has_y;
+    w.fill = std::max<int64_t>(has_z ? bzs : by, 1);
+    w.yout = std::max<int64_t>(by, 1);
+    return w;
+}
+
+// nb_cap > 0 bounds the realizations per launch (ss3d_realize_device_noalloc: what its scratch
+// windows hold).
 int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nreal, int nx, int ny,
-                 int nz, void* dout, double* dstats) {
+                 int nz, void* dout, double* dstats, int nb_cap = 0) {
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
     if (lx > kMaxFastLog || ly > kMaxFastLog || lz > kMaxFastLog)
@@ -999,7 +1069,8 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     a.txp = sp->rng[igauss].txp.as<uint32_t>();
     const bool has_y = npy > 1, has_z = npz > 1;
     const int64_t ncrop = (int64_t)nx * ny * nz;
-    const int nb = batch_size(sp, nx, ny, nz, SS3D_FP32, nreal);
+    const int nb = nb_cap > 0 ? std::min(nb_cap, batch_size(sp, nx, ny, nz, SS3D_FP32, nreal))
+                              : batch_size(sp, nx, ny, nz, SS3D_FP32, nreal);
     const int hc = npx >= 2 ? npx / 2 : 1, hcs = std::max(2, hc);
     const int64_t bz = (int64_t)hcs * npy * nz, by = (int64_t)hcs * ny * nz;
     const float2* tw32 = c->tw32n.as<float2>();
@@ -1061,12 +1132,12 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     px.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
 
     const bool staged = has_y || has_z;  // a fill pass followed by further passes
-    const int64_t pipe_elems = has_z ? bzs : by;  // output of the fill pass
+    const Fp32Work wk = fp32_work(sp, nx, ny, nz);
     hipStream_t s2 = c->pipeline ? L.stream2 : L.stream;  // stream of the passes after the fill
     const bool pipelined = staged && c->pipeline;
     if (staged) {
-        for (int i = 0; i < 2; ++i) SS3D_TRY(ensure2(L, L.pipe[i], (size_t)std::max<int64_t>(pipe_elems, 1) * nb * 8));
-        if (has_z && has_y) SS3D_TRY(ensure2(L, L.wb, (size_t)std::max<int64_t>(by, 1) * nb * 8));
+        for (int i = 0; i < 2; ++i) SS3D_TRY(ensure2(L, L.pipe[i], (size_t)wk.fill * nb * 8));
+        if (wk.has_wb) SS3D_TRY(ensure2(L, L.wb, (size_t)wk.yout * nb * 8));
     }
     const int nbat = (nreal + nb - 1) / nb;
     // fusion keeps the y pass's own radix plan (results do not depend on the schedule): the fill's
@@ -1209,6 +1280,130 @@ int ss3d_realize_device(ss3d_context* c, const ss3d_spectrum* spc, int iseed, in
         void* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * 8;
         double* st = (dev_stats ? dev_stats : c->main.stats.as<double>()) + 3 * (size_t)done;
         SS3D_TRY(enqueue_batch(c, c->main, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, prec, out, st));
+    }
+    return SS3D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Scratch layout of ss3d_realize_device_noalloc: realization bases and (when the caller passes
+// no statistics buffer) statistics for `chunk` realizations per enqueue, then per batch member
+// the block statistics and the pass buffers, for nb members per launch.
+struct ScratchPlan {
+    int nb = 0, chunk = 0;
+    size_t rbase = 0, stats = 0, partial = 0, w1 = 0, w2 = 0, w3 = 0;
+};
+
+int plan_scratch(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal, bool own_stats, ScratchPlan& q) {
+    const int64_t npy = sp->npy, npz = sp->npz;
+    int64_t blocks, per1, per2 = 0, per3 = 0;  // blocks with statistics, buffer bytes per member
+    if (prec == SS3D_FP32) {
+        const Fp32Work w = fp32_work(sp, nx, ny, nz);
+        blocks = (int64_t)ny * nz;  // x pass: at most one block per row
+        if (w.staged) {
+            per1 = per2 = w.fill * 8;
+            per3 = w.has_wb ? w.yout * 8 : 0;
+        } else {
+            per1 = 0;
+        }
+    } else {
+        // last pass: x rows (npy npz), y tiles (<= nx npz) or z tiles (<= nx ny)
+        blocks = std::max<int64_t>(npy * npz, std::max<int64_t>((int64_t)nx * npz, (int64_t)nx * ny));
+        per1 = (int64_t)nx * npy * npz * 16;
+        per2 = std::max<int64_t>((int64_t)nx * ny * npz, 1) * 16;
+    }
+    const int64_t per = (int64_t)up256(blocks * 4 * sizeof(double)) + (int64_t)up256(per1) + (int64_t)up256(per2) +
+                        (int64_t)up256(per3);
+    q.chunk = std::min(nreal, 4096);
+    const size_t fixed = up256((size_t)q.chunk * 4) + (own_stats ? up256((size_t)q.chunk * 3 * sizeof(double)) : 0);
+    const int64_t room = (int64_t)kScratchBytes - (int64_t)fixed;
+    if (room < per)
+        return fail(SS3D_ETOOBIG,
+                    "ss3d_realize_device_noalloc: one realization of this grid needs %lld bytes of scratch, the "
+                    "scratch holds %zu (use ss3d_realize_device)",
+                    (long long)(per + (int64_t)fixed), kScratchBytes);
+    q.nb = (int)std::min<int64_t>({room / per, (int64_t)batch_size(sp, nx, ny, nz, prec, nreal), (int64_t)q.chunk});
+    q.rbase = (size_t)q.chunk * 4;
+    q.stats = own_stats ? (size_t)q.chunk * 3 * sizeof(double) : 0;
+    q.partial = (size_t)blocks * 4 * sizeof(double) * q.nb;
+    q.w1 = (size_t)per1 * q.nb;
+    q.w2 = (size_t)per2 * q.nb;
+    q.w3 = (size_t)per3 * q.nb;
+    return SS3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ss3d_realize_device_noalloc(ss3d_context* c, const ss3d_spectrum* spc, int iseed, int igauss, int64_t r0,
+                                int nreal, int nx, int ny, int nz, int prec, void* dev_out, double* dev_stats) {
+    SS3D_TRY(check_realize(c, spc, igauss, r0, nreal, nx, ny, nz, prec));
+    if (nreal == 0) return SS3D_OK;
+    if (!dev_out) return fail(SS3D_EINVAL, "null device output");
+    if (!c->scratch) return fail(SS3D_EHIP, "no scratch address on device %d", c->device);
+    ss3d_spectrum* sp = const_cast<ss3d_spectrum*>(spc);
+    if (!sp->rng[igauss].ready)
+        return fail(SS3D_EINVAL,
+                    "ss3d_realize_device_noalloc: the spectrum's random-stream tables for igauss=%d are not built; "
+                    "one ss3d_realize or ss3d_realize_device call with this igauss builds them",
+                    igauss);
+    ScratchPlan q;
+    SS3D_TRY(plan_scratch(sp, nx, ny, nz, prec, nreal, dev_stats == nullptr, q));
+    HIP_TRY(hipSetDevice(c->device));
+    // windows of the scratch array; buffers this call does not use get none
+    Lane& L = c->sl;
+    for (DevBuf* b : {&L.wa, &L.wb, &L.pipe[0], &L.pipe[1], &L.partial, &L.partial2, &L.rbase, &L.stats, &L.dout,
+                      &L.diag})
+        b->set_view(nullptr, 0);
+    char* base = static_cast<char*>(c->scratch);
+    size_t off = 0;
+    auto take = [&](DevBuf& b, size_t bytes) {
+        b.set_view(base + off, bytes);
+        off += up256(bytes);
+    };
+    take(L.rbase, q.rbase);
+    if (!dev_stats) take(L.stats, q.stats);
+    take(L.partial, q.partial);
+    if (prec == SS3D_FP32) {
+        take(L.pipe[0], q.w1);
+        take(L.pipe[1], q.w2);
+        take(L.wb, q.w3);
+    } else {
+        take(L.wa, q.w1);
+        take(L.wb, q.w2);
+    }
+    if (off > kScratchBytes) return fail(SS3D_ETOOBIG, "scratch layout of %zu bytes exceeds %zu", off, kScratchBytes);
+    // one stream, no profiling events (event creation is left to the allocating entry points)
+    struct Keep {
+        ss3d_context* c;
+        bool pipeline, profiling;
+        ~Keep() {
+            c->pipeline = pipeline;
+            c->profiling = profiling;
+        }
+    } keep{c, c->pipeline, c->profiling};
+    c->pipeline = false;
+    c->profiling = false;
+    const int64_t ncrop = (int64_t)nx * ny * nz;
+    const size_t esz = prec == SS3D_FP64 ? 8 : 4;
+    for (int done = 0; done < nreal; done += q.chunk) {
+        const int cnt = std::min(q.chunk, nreal - done);
+        char* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * esz;
+        double* st = dev_stats ? dev_stats + 3 * (size_t)done : L.stats.as<double>();
+        if (prec == SS3D_FP32) {
+            SS3D_TRY(enqueue_fp32(c, L, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, out, st, q.nb));
+        } else {
+            for (int k = 0; k < cnt; k += q.nb) {
+                const int m = std::min(q.nb, cnt - k);
+                SS3D_TRY(enqueue_batch(c, L, sp, iseed, igauss, r0 + done + k, m, nx, ny, nz, prec,
+                                       out + (size_t)k * ncrop * esz, st + 3 * (size_t)k));
+            }
+        }
     }
     return SS3D_OK;
 }
