@@ -193,7 +193,7 @@ __global__ __launch_bounds__((1 << L) / (1 << el(L)) * strided_P(L)) void stride
                         amp = sqrt(sr);
                     }
                     const int64_t row = j + (int64_t)a.npy * k;
-                    a.amp64_out[i0 + (int64_t)a.npx * row] = amp;
+                    if (a.amp64_out) a.amp64_out[i0 + (int64_t)a.npx * row] = amp;
                     if (a.amp32h_out && i0 < a.hx) a.amp32h_out[i0 + (int64_t)a.hxa * row] = (float)(amp * kAmp32Scale);
                 }
         }

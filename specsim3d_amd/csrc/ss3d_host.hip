@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -159,8 +160,21 @@ struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
     bool view = false;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { swap_with(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        swap_with(o);
+        return *this;
+    }
     ~DevBuf() {
         if (p && !view) (void)hipFree(p);
+    }
+    void swap_with(DevBuf& o) noexcept {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        std::swap(view, o.view);
     }
     void set_view(void* base, size_t bytes) {
         p = base;
@@ -185,6 +199,61 @@ struct DevBuf {
         return static_cast<T*>(p);
     }
 };
+
+// Device buffers handed back by the request loop's evicted spectra and reused by the spectra it
+// builds next: a cache miss takes memory from here instead of calling hipMalloc, and an eviction
+// calls no hipFree (which waits for the whole device).
+struct BufPool {
+    std::vector<DevBuf> bufs;
+    int64_t bytes = 0;
+    // the smallest pooled buffer of at least `need` bytes and at most about twice that (a small
+    // table does not take an amplitude array), or -1
+    int best(size_t need) const {
+        const size_t most = 2 * need + (size_t(1) << 20);
+        int k = -1;
+        for (int i = 0; i < (int)bufs.size(); ++i)
+            if (bufs[i].cap >= need && bufs[i].cap <= most && (k < 0 || bufs[i].cap < bufs[k].cap)) k = i;
+        return k;
+    }
+    // moves a pooled buffer of at least `need` bytes into b (b's own buffer, if any, joins the pool)
+    bool take(DevBuf& b, size_t need) {
+        const int k = best(need);
+        if (k < 0) return false;
+        bytes -= (int64_t)bufs[k].cap;
+        b.swap_with(bufs[k]);
+        if (bufs[k].p) {
+            bytes += (int64_t)bufs[k].cap;
+        } else {
+            bufs[k] = std::move(bufs.back());
+            bufs.pop_back();
+        }
+        return true;
+    }
+    void put(DevBuf& b) {
+        if (!b.p || b.view) return;
+        bytes += (int64_t)b.cap;
+        bufs.emplace_back();
+        bufs.back().swap_with(b);
+    }
+    // frees the largest pooled buffer (the last resort of the request loop's memory budget)
+    bool free_largest() {
+        if (bufs.empty()) return false;
+        int k = 0;
+        for (int i = 1; i < (int)bufs.size(); ++i)
+            if (bufs[i].cap > bufs[k].cap) k = i;
+        bytes -= (int64_t)bufs[k].cap;
+        bufs[k] = std::move(bufs.back());  // the buffer moved out lands in bufs.back() and is freed below
+        bufs.pop_back();
+        return true;
+    }
+};
+
+// b holds at least `bytes`: as it is, from the pool, or newly allocated.
+int grab(BufPool* pool, DevBuf& b, size_t bytes, hipStream_t s) {
+    if (b.cap >= bytes) return SS3D_OK;
+    if (pool && pool->take(b, bytes)) return SS3D_OK;
+    return b.ensure(bytes, s);
+}
 
 // One ordered lane of device work: a stream (plus a second one for the FP32 pipeline) and the
 // work buffers a spectrum build or a realization call needs.  A context has one lane; a server
@@ -291,6 +360,8 @@ struct ss3d_spectrum {
     int npx = 0, npy = 0, npz = 0, hx = 0, hxa = 0;
     DevBuf amp64, amp32h, diagd;
     RngTables rng[2];
+    bool has64 = true;          // amp64 written (the request loop builds FP32-only spectra without it)
+    BufPool* pool = nullptr;    // request-loop spectra: buffers come from (and return to) this pool
     ss3d_spec_diag diag{};
     // the model this spectrum was made for (ss3d_spectrum_create / load): the key of its file
     int key_nst = 0;
@@ -387,8 +458,8 @@ int build_rng(ss3d_spectrum* sp, int igauss) {
         }
     }
     hipStream_t s = sp->ctx->stream;
-    SS3D_TRY(t.txp.ensure(tx.size() * 4, s));
-    SS3D_TRY(t.rowf.ensure(rowf.size() * 4, s));
+    SS3D_TRY(grab(sp->pool, t.txp, tx.size() * 4, s));
+    SS3D_TRY(grab(sp->pool, t.rowf, rowf.size() * 4, s));
     HIP_TRY(hipMemcpy(t.txp.p, tx.data(), tx.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.rowf.p, rowf.data(), rowf.size() * 4, hipMemcpyHostToDevice));
     t.ready = true;
@@ -702,8 +773,11 @@ int64_t ss3d_realization_state(int iseed, int npx, int npy, int npz, int igauss,
 
 // Enqueues the whole spectrum build on lane L (no host synchronisation); the diagnostics land in
 // sp->diagd on the device.
+// pool: buffers from (request loop) or nullptr; with64 = false skips the FP64 amplitudes (a
+// spectrum for FP32 realizations only).
 static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, int nst, double nugget, double dx,
-                          double dy, double dz, int npx, int npy, int npz, std::unique_ptr<ss3d_spectrum>& sp) {
+                          double dy, double dz, int npx, int npy, int npz, std::unique_ptr<ss3d_spectrum>& sp,
+                          BufPool* pool = nullptr, bool with64 = true) {
     SS3D_TRY(check_structures(st, nst));
     SS3D_TRY(check_grid(npx, npy, npz));
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
@@ -720,11 +794,13 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     sp->hxa = (sp->hx + 1) & ~1;
     const int64_t ntot = (int64_t)npx * npy * npz;
     const int64_t nh = (int64_t)sp->hxa * npy * npz;
-    SS3D_TRY(sp->amp64.ensure(ntot * sizeof(double), L.stream));
-    SS3D_TRY(sp->amp32h.ensure(nh * sizeof(float), L.stream));
+    sp->pool = pool;
+    sp->has64 = with64;
+    if (with64) SS3D_TRY(grab(pool, sp->amp64, ntot * sizeof(double), L.stream));
+    SS3D_TRY(grab(pool, sp->amp32h, nh * sizeof(float), L.stream));
     HIP_TRY(hipMemsetAsync(sp->amp32h.p, 0, nh * sizeof(float), L.stream));  // padding column stays 0
     SS3D_TRY(L.wa.ensure(ntot * 2 * sizeof(double), L.stream));
-    SS3D_TRY(sp->diagd.ensure(4 * sizeof(double), L.stream));
+    SS3D_TRY(grab(pool, sp->diagd, 4 * sizeof(double), L.stream));
 
     PassArgs a = base_args(sp.get());
     a.nst = nst;
@@ -742,7 +818,7 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
         std::memcpy(a.st[s].rot, rot, sizeof rot);
         c0 = c0 + st[s].sill * 1.0;
     }
-    a.amp64_out = sp->amp64.as<double>();
+    a.amp64_out = with64 ? sp->amp64.as<double>() : nullptr;
     a.amp32h_out = sp->amp32h.as<float>();
     const int emax = 8192;
     const bool has_y = npy > 1, has_z = npz > 1;
@@ -908,6 +984,7 @@ int enqueue_batch(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int ig
                   int nz, int prec, void* dout, double* dstats) {
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
+    if (!sp->has64) return fail(SS3D_EINVAL, "spectrum built without FP64 amplitudes");
     SS3D_TRY(build_rng(sp, igauss));
     SS3D_TRY(L.rbase.ensure((size_t)nb * 4, L.stream));
     {

@@ -425,63 +425,93 @@ contains
     vario_ok = .true.
   end subroutine vario_read
 
-  ! Experimental variograms of every realization along the directions of the variogram file,
-  ! their ensemble mean and sd and the model variogram: the report and the <base>_vario.csv table
-  ! of vario_report (f90:434-529), with the pair sums computed on the GPU (ss3d_variogram).
+  ! Total sill of the model: the nugget plus every structure's sill, added in structure order
+  ! (the value the reference's report takes as C(0)).
+  real(dp) function model_sill()
+    integer :: is
+    model_sill = nugget
+    do is = 1, nst
+      model_sill = model_sill + st(is)%sill
+    end do
+  end function model_sill
+
+  ! The rows of one direction of the variogram table: the lags 1..nlag whose shift keeps pairs
+  ! inside the grid, each with its distance, pair count, model semivariogram (sill - C(h)) and the
+  ! ensemble mean and sd of the realizations' values; nrow of them.
+  subroutine vario_rows(id, sill, nrow, lagv, dist, npair, gmod, gmean, gsdev)
+    integer, intent(in) :: id
+    real(dp), intent(in) :: sill
+    integer, intent(out) :: nrow
+    integer, intent(out) :: lagv(:)
+    integer(8), intent(out) :: npair(:)
+    real(dp), intent(out) :: dist(:), gmod(:), gmean(:), gsdev(:)
+    integer :: lag
+    integer(8) :: np
+    real(dp) :: shift(3)
+    nrow = 0
+    do lag = 1, dirs(4, id)
+      np = ss3d_npairs(int(nx, c_int), int(ny, c_int), int(nz, c_int), int(lag*dirs(1, id), c_int), &
+                       int(lag*dirs(2, id), c_int), int(lag*dirs(3, id), c_int))
+      if (np <= 0) cycle
+      nrow = nrow + 1
+      shift = real(lag*dirs(1:3, id), dp)*spc
+      lagv(nrow) = lag
+      npair(nrow) = np
+      dist(nrow) = sqrt(shift(1)*shift(1) + shift(2)*shift(2) + shift(3)*shift(3))
+      gmod(nrow) = sill - ss3d_covariance(st, int(nst, c_int), nugget, shift(1), shift(2), shift(3))
+      gmean(nrow) = sum(gam(lag, id, 1:nreal))/real(nreal, dp)
+      gsdev(nrow) = 0.0_dp
+      if (nreal > 1) gsdev(nrow) = sqrt(sum((gam(lag, id, 1:nreal) - gmean(nrow))**2)/real(nreal - 1, dp))
+    end do
+  end subroutine vario_rows
+
+  ! The variogram report (f90:434-529): <base>_vario.csv with one row per direction and lag (and the
+  ! realizations' values when the file asks for them), and per direction the RMS and largest
+  ! deviation of the ensemble mean from the model over the rising part of the model
+  ! (gamma_model < 0.95 sill).  The pair sums come from the GPU (ss3d_variogram, on each chunk).
   subroutine vario_report(vfile)
     character(len=*), intent(in) :: vfile
-    integer :: id, il, r, nfit, u
-    real(dp) :: c0, gm, gsd, gth, h, rms, mx, e, hx, hy, hz
-    integer(8) :: np
+    integer :: id, k, nrow, nfit, u, nper, r
+    integer, allocatable :: lagv(:)
+    integer(8), allocatable :: npair(:)
+    real(dp), allocatable :: dist(:), gmod(:), gmean(:), gsdev(:), dev(:)
+    logical, allocatable :: rising(:)
+    real(dp) :: sill, rmsd, maxd
     character(len=512) :: vname
     if (.not. vario_ok) then
       write(*,*) 'cannot open variogram file: ', vfile; return
     end if
-    c0 = nugget
-    do i = 1, nst
-      c0 = c0 + st(i)%sill
-    end do
+    sill = model_sill()
+    nper = 0
+    if (iper == 1) nper = nreal
     vname = trim(outbase)//'_vario.csv'
     open(newunit=u, file=trim(vname), status='replace', action='write')
     write(u, '(a)', advance='no') 'dir,sx,sy,sz,lag,h,npairs,gamma_model,gamma_mean,gamma_sd'
-    if (iper == 1) then
-      do r = 1, nreal
-        write(u, '(a,i0)', advance='no') ',gamma_r', r
-      end do
-    end if
+    if (nper > 0) write(u, '(*(a,i0))', advance='no') (',gamma_r', r, r = 1, nper)
     write(u, '(a)') ''
     write(*, '(a)') ' variograms (Fortran): direction  lags  RMS(mean-model)  max|err|   [rising part, ' // &
                     'gamma_model < 0.95 sill]'
     do id = 1, ndir
-      rms = 0.0_dp; mx = 0.0_dp; nfit = 0
-      do il = 1, dirs(4, id)
-        np = ss3d_npairs(int(nx, c_int), int(ny, c_int), int(nz, c_int), int(il*dirs(1, id), c_int), &
-                         int(il*dirs(2, id), c_int), int(il*dirs(3, id), c_int))
-        if (np == 0) cycle
-        hx = il*dirs(1, id)*spc(1); hy = il*dirs(2, id)*spc(2); hz = il*dirs(3, id)*spc(3)
-        h = sqrt(hx*hx + hy*hy + hz*hz)
-        gth = c0 - ss3d_covariance(st, int(nst, c_int), nugget, hx, hy, hz)
-        gm = sum(gam(il, id, 1:nreal))/real(nreal, dp)
-        if (nreal > 1) then
-          gsd = sqrt(sum((gam(il, id, 1:nreal) - gm)**2)/real(nreal - 1, dp))
-        else
-          gsd = 0.0_dp
-        end if
-        write(u, '(i0,3(",",i0),",",i0,",",es14.6,",",i0,3(",",es14.6))', advance='no') &
-              id, dirs(1, id), dirs(2, id), dirs(3, id), il, h, np, gth, gm, gsd
-        if (iper == 1) then
-          do r = 1, nreal
-            write(u, '(",",es14.6)', advance='no') gam(il, id, r)
-          end do
-        end if
+      k = max(1, dirs(4, id))
+      allocate(lagv(k), npair(k), dist(k), gmod(k), gmean(k), gsdev(k))
+      call vario_rows(id, sill, nrow, lagv, dist, npair, gmod, gmean, gsdev)
+      do k = 1, nrow
+        write(u, '(i0,3(",",i0),",",i0,",",es14.6,",",i0,3(",",es14.6))', advance='no') id, dirs(1:3, id), &
+          lagv(k), dist(k), npair(k), gmod(k), gmean(k), gsdev(k)
+        if (nper > 0) write(u, '(*(",",es14.6))', advance='no') gam(lagv(k), id, 1:nper)
         write(u, '(a)') ''
-        if (gth < 0.95_dp*c0) then
-          e = gm - gth
-          rms = rms + e*e; mx = max(mx, abs(e)); nfit = nfit + 1
-        end if
       end do
-      if (nfit > 0) rms = sqrt(rms/real(nfit, dp))
-      write(*, '(a,i3,3i4,i6,2f12.5)') '   dir', id, dirs(1, id), dirs(2, id), dirs(3, id), dirs(4, id), rms, mx
+      rising = gmod(1:nrow) < 0.95_dp*sill
+      dev = gmean(1:nrow) - gmod(1:nrow)
+      nfit = count(rising)
+      rmsd = 0.0_dp
+      maxd = 0.0_dp
+      if (nfit > 0) then
+        rmsd = sqrt(sum(dev*dev, mask=rising)/real(nfit, dp))
+        maxd = maxval(abs(dev), mask=rising)
+      end if
+      write(*, '(a,i3,3i4,i6,2f12.5)') '   dir', id, dirs(1:3, id), dirs(4, id), rmsd, maxd
+      deallocate(lagv, npair, dist, gmod, gmean, gsdev)
     end do
     close(u)
     write(*, '(a,a)') ' variogram table: ', trim(vname)

@@ -1,12 +1,18 @@
 """Request loop (continuous batching): every result is the same alone and in the mix, the
 spectral-density cache is hit by repeated models, FP64 spot checks stay within the FP32
-tolerance, and a request the engine cannot take fails alone."""
+tolerance, and a request the engine cannot take fails alone.  A mix that holds requests of the
+benchmark's size (256^3 grid, 512^3 periodic) is checked against the reference: the FP64 request
+against the reference program's own realizations (tests/golden/c2, 1e-10), the FP32 one within
+1e-5 relative RMS of them, and a small FP64 request against the oracle and its golden fixture."""
 import sys
 
 import numpy as np
 import pytest
 
+import golden_cases as G
+import oracle_ffi as O
 import specsim3d_amd as S
+from test_gpu_configs import check_against_fixture, fixture, rel_max, rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -84,3 +90,49 @@ def test_failing_request_does_not_stop_others():
                     done[c["id"]] = c
     assert done[1]["status"] == 2  # SS3D_ETOOBIG
     assert done[2]["status"] == 0
+
+
+def test_benchmark_sized_requests_in_the_mix_match_the_reference():
+    meta, fx = fixture("c2")
+    big = [tuple(v) for v in meta["structures"]]
+    nxb, nyb, nzb = meta["grid"]
+    small = G.cases()["exp_nug_gauss"]
+    # (structures, nugget, grid, seed, nreal, igauss, precision, spot check)
+    jobs = [(big, meta["nugget"], (nxb, nyb, nzb), meta["iseed"], 2, meta["igauss"], S.FP64, False),
+            (big, meta["nugget"], (nxb, nyb, nzb), meta["iseed"], 2, meta["igauss"], S.FP32, True),
+            (small["structures"], small["nugget"], tuple(small["grid"]), small["iseed"], small["nreal"],
+             small["igauss"], S.FP64, False),
+            (big, meta["nugget"], (nxb, nyb, nzb), 777, 8, 1, S.FP32, False),
+            ([(2, 0.7, 90, 35, 12, 10, -5, 0)], 0.3, (nxb, nyb, nzb), 778, 4, 1, S.FP32, False)]
+    say("[server] benchmark-sized mix: alone")
+    with S.Engine(0) as eng:
+        ref = []
+        for sts, nug, grid, seed, nreal, ig, prec, _ in jobs:
+            npg = S.periodic_grid(sts, 1, 1, 1, *grid, 1)
+            sp = eng.spectrum(sts, nug, 1, 1, 1, *npg)
+            ref.append(eng.realize(sp, seed, ig, 0, nreal, *grid, prec))
+            sp.close()
+        say("[server] benchmark-sized mix: in the mix")
+        srv = S.Server(eng, nslots=4)
+        outs, done = [], {}
+        try:
+            for i, (sts, nug, grid, seed, nreal, ig, prec, spot) in enumerate(jobs):
+                out = np.empty((nreal, grid[2], grid[1], grid[0]), dtype=np.float32 if prec == S.FP32 else np.float64)
+                outs.append(out)
+                srv.submit(S.Server.request(i, sts, nug, 1, 1, 1, *grid, 1, seed, nreal, ig, prec, spot_check=spot,
+                                            host_out=out), keep=out)
+            while len(done) < len(jobs):
+                for c in srv.poll(timeout=1.0):
+                    done[c["id"]] = c
+        finally:
+            srv.close()
+    for i in range(len(jobs)):
+        assert done[i]["status"] == 0, done[i]
+        assert np.array_equal(outs[i], ref[i][0]), f"request {i} differs from its result alone"
+        assert done[i]["mean0"] == ref[i][1][0] and done[i]["var0"] == ref[i][2][0]
+    check_against_fixture(outs[0], fx, 1e-10, rel_max)   # FP64 256^3: the reference's realizations
+    check_against_fixture(outs[1], fx, 1e-5, rel_rms)    # FP32 256^3: within the FP32 bar of them
+    assert 0 <= done[1]["spot_rel_rms"] < 1e-5
+    o = O.simulate(*G.args(small))
+    assert rel_max(outs[2], o["res"]) < 1e-10 and rel_max(outs[2], G.field("exp_nug_gauss")) < 1e-10
+    assert done[3]["cache_hit"] == 1  # same model and grid as requests 0 and 1
