@@ -67,13 +67,18 @@ class Hip:
         self.L.hipDeviceSynchronize()
 
 
+REGION = "timed FP32 region"
+
+
 class Regions:
-    """rocprofv3 --selected-regions brackets (roctxProfilerResume / roctxProfilerPause of the
-    rocprofiler-sdk ROCTx library): a profile of this script then holds exactly the dispatches of
-    the timed FP32 region.  Without the profiler the calls do nothing."""
+    """A ROCTx range named REGION around the timed FP32 region (rocprofiler-sdk ROCTx library), so
+    that `rocprofv3 --kernel-trace --marker-trace` of this script marks which dispatches were timed
+    (see --trace-summary); the region's CLOCK_MONOTONIC bounds, the clock of rocprofv3's
+    timestamps, also go into the JSON line.  Without the profiler the calls do nothing."""
 
     def __init__(self):
         self.L = None
+        self.bounds = [None, None]
         for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
             try:
                 self.L = ctypes.CDLL(name)
@@ -81,17 +86,73 @@ class Regions:
             except OSError:
                 continue
         if self.L is not None:
-            for f in (self.L.roctxProfilerResume, self.L.roctxProfilerPause):
-                f.argtypes = [ctypes.c_uint64]
-                f.restype = ctypes.c_int
+            self.L.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            self.L.roctxRangePushA.restype = ctypes.c_int
+            self.L.roctxRangePop.restype = ctypes.c_int
 
     def resume(self):
+        self.bounds[0] = time.monotonic_ns()
         if self.L is not None:
-            self.L.roctxProfilerResume(0)
+            self.L.roctxRangePushA(REGION.encode())
 
     def pause(self):
         if self.L is not None:
-            self.L.roctxProfilerPause(0)
+            self.L.roctxRangePop()
+        self.bounds[1] = time.monotonic_ns()
+
+
+# pass kind of the library's profile -> kernel of that pass (rocprofv3 names)
+KIND_KERNELS = {"real_zy_fp32": r"fill_load_kernel<", "real_z_fp32": r"fast::strided_kernel<\d+, \d+, 3, 2,",
+                "real_y_fp32": r"fast::strided_kernel<\d+, \d+, 2, 1,", "real_x_c2r_fp32": r"fast::c2r_kernel<"}
+
+
+def trace_summary(trace_csv, marker_csv, bench_json, command):
+    """Markdown summary of the kernels of the timed FP32 region: the dispatches of a rocprofv3
+    kernel trace that lie inside the ROCTx range REGION (or the JSON line's monotonic bounds), with
+    the dominant kernel's average duration beside the HIP-event figure of the same run."""
+    import csv
+
+    with open(bench_json) as fh:
+        line = json.loads([ln for ln in fh.read().splitlines() if ln.startswith("{")][-1])
+    t0, t1 = line.get("timed_region_monotonic_ns", [None, None])
+    if marker_csv and os.path.exists(marker_csv):
+        with open(marker_csv) as fh:
+            for r in csv.DictReader(fh):
+                if r["Function"] == REGION:
+                    t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+    with open(trace_csv) as fh:
+        rows = [r for r in csv.DictReader(fh) if t0 <= int(r["Start_Timestamp"]) and int(r["End_Timestamp"]) <= t1]
+    stats = {}
+    for r in rows:
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+        n, tot = stats.get(r["Kernel_Name"], (0, 0.0))
+        stats[r["Kernel_Name"]] = (n + 1, tot + d)
+    total = sum(v[1] for v in stats.values()) or 1.0
+    rf = line["roofline"]
+    out = ["# rocprofv3 kernel trace -- the timed FP32 region of bench.py (256^3 in 512^3, 64 realizations per step)", "",
+           "Workload: 256^3 output grid in a 512^3 periodic grid, exponential sill 0.7 + nugget 0.3, ranges 80/40/10, "
+           "az 30, dip 15, ipad=1, igauss=1; FP32 output in HBM.  Dispatches: those inside the ROCTx range "
+           f"'{REGION}' that bench.py pushes around its timed steps ({line['steps']} steps after "
+           f"{line['warmup']} warm-up steps); nothing else of the run is counted.", "",
+           "Commands (repository root on the GPU host, after `cd /tmp && export TMPDIR=/tmp`):", "",
+           "    " + command, f"    python3 bench.py --trace-summary {trace_csv} --marker-csv {marker_csv} "
+           f"--bench-json {bench_json}", "",
+           f"bench.py in the same run: {line['value']} Gcells/s, {line['ms_per_step']} ms per step.", "",
+           "| kernel | calls | total ms | average ms | % |", "|---|---|---|---|---|"]
+    for k, (n, tot) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"| `{k}` | {n} | {tot:.2f} | {tot / n:.4f} | {100 * tot / total:.2f} |")
+    pat = re.compile(KIND_KERNELS.get(rf["kernel"], r"^$"))
+    dom = [(k, v) for k, v in stats.items() if pat.search(k)]
+    if dom:
+        k, (n, tot) = max(dom, key=lambda kv: kv[1][1])
+        avg = tot / n
+        ach = rf["algorithmic_bytes_per_launch"] / (avg * 1e-3) / 1e9
+        out += ["", f"Dominant kernel ({rf['kernel']}): {n} dispatches, average {avg:.4f} ms in the trace against "
+                    f"{rf['ms_per_launch']:.4f} ms from the library's HIP events (difference "
+                    f"{100 * (avg / rf['ms_per_launch'] - 1):+.1f}%).  Algorithmic bytes per launch "
+                    f"{rf['algorithmic_bytes_per_launch'] / 1e9:.3f} GB: {ach:.0f} GB/s = {ach / rf['peak']:.4f} of "
+                    f"{rf['peak']:.0f} GB/s from the trace, {rf['frac']} in the JSON line."]
+    return "\n".join(out) + "\n"
 
 
 def host_cpu():
@@ -550,6 +611,13 @@ def main():
                     help="seconds of arrivals at 200 req/s (the overload run)")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving legs and print their JSON (not the benchmark line)")
+    ap.add_argument("--trace-summary", default="",
+                    help="summarize a rocprofv3 kernel_trace.csv of this script (no GPU work) and exit")
+    ap.add_argument("--marker-csv", default="", help="with --trace-summary: the run's marker_api_trace.csv")
+    ap.add_argument("--bench-json", default="", help="with --trace-summary: the run's JSON line")
+    ap.add_argument("--trace-command", default="rocprofv3 --kernel-trace --marker-trace --stats --output-format csv "
+                                               "-d gpurun_out/prof_c2 -o c2 -- python3 -u bench.py",
+                    help="with --trace-summary: the command that made the trace (quoted in the summary)")
     ap.add_argument("--serve-slots", type=int, default=2,
                     help="lanes of the request loop (2: a spectrum build or a request overlaps another request; "
                          "more lanes thrash the caches shared by their amplitude tables)")
@@ -563,6 +631,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
+    if args.trace_summary:
+        sys.stdout.write(trace_summary(args.trace_summary, args.marker_csv, args.bench_json, args.trace_command))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Realizations are independent: a multi-GPU run would give each GPU a range of them.", file=sys.stderr)
@@ -699,6 +770,7 @@ def main():
         "conditioning": conditioning,
         "latency_ms": ({"p50": serving["sustained"]["p50_ms"], "p99": serving["sustained"]["p99_ms"],
                         "rate_req_s": args.serve_rate, "serving": serving} if serving else None),
+        "timed_region_monotonic_ns": regions.bounds,
         "roofline": {"kernel": kind, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kind),
                      "algorithmic_bytes_per_launch": byt / nl, "ms_per_launch": ms / nl},
