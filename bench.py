@@ -365,6 +365,20 @@ def uneven_batch(S, eng, seed, njobs=32, nlanes=4, reps=5):
                float(rng.uniform(0, 180)), float(rng.uniform(-30, 30)), 0.0)]
         reqs.append(S.Server.request(i, st, nug, 1, 1, 1, nx, ny, nz, 1, int(rng.integers(1, 2**31 - 1)),
                                      int(rng.integers(1, 17)), 1, S.FP32))
+    # untuned: a fresh context (empty tuning table: largest radix 32, default tiles), no batch tuning
+    with S.Engine(0) as e0:
+        e0.set_option(S.OPT_BATCH_AUTOTUNE, 0)
+        b0 = S.Batch(e0, reqs, nlanes=nlanes)
+        try:
+            b0.run(graph=True)
+            e0.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                b0.run(graph=True)
+            e0.synchronize()
+            untuned = (time.perf_counter() - t0) / reps
+        finally:
+            b0.close()
     t0 = time.perf_counter()
     b = S.Batch(eng, reqs, nlanes=nlanes)
     eng.synchronize()
@@ -381,10 +395,22 @@ def uneven_batch(S, eng, seed, njobs=32, nlanes=4, reps=5):
             out[mode] = (time.perf_counter() - t0) / reps
     finally:
         b.close()
+    # the plans and tiles the batch's autotuning left for every pass length of its grids
+    picks = {}
+    for r in reqs:
+        g = S.periodic_grid([(r.st[0].itype, r.st[0].sill, r.st[0].ahmax, r.st[0].ahmin, r.st[0].avert, r.st[0].az,
+                              r.st[0].dip, r.st[0].plg)], 1, 1, 1, r.nx, r.ny, r.nz, 1)
+        lx, ly, lz = (int(v).bit_length() - 1 for v in g)
+        for name, kind, L in (("real_z_fp32", 6, lz), ("real_y_fp32", 7, ly), ("real_x_c2r_fp32", 8, max(lx - 1, 0))):
+            t = eng.tuning_get_tile(kind, L)
+            picks[f"{name} 2^{L}"] = f"radix 2^{t['rm']}, tile {t['tile']}{' padded' if t['lds_pad'] else ''}" + \
+                (" (variant 1)" if t["variant"] else "")
     return {"jobs": njobs, "lanes": nlanes, "cells_per_batch": b.cells, "reps": reps,
             "graph_gcells_s": round(b.cells / out["graph"] / 1e9, 3), "graph_ms": round(out["graph"] * 1e3, 3),
             "eager_gcells_s": round(b.cells / out["eager"] / 1e9, 3), "eager_ms": round(out["eager"] * 1e3, 3),
             "create_s": round(t_create, 2),
+            "untuned_graph_gcells_s": round(b.cells / untuned / 1e9, 3),
+            "tuned_picks": dict(sorted(picks.items())),
             "distinct_periodic_grids": len({S.periodic_grid([(r.st[0].itype, r.st[0].sill, r.st[0].ahmax,
                                                               r.st[0].ahmin, r.st[0].avert, r.st[0].az, r.st[0].dip,
                                                               r.st[0].plg)], 1, 1, 1, r.nx, r.ny, r.nz, 1)
@@ -403,13 +429,26 @@ def large_job(S, eng, hip, nreal=32, checks=(0, 31)):
     nx, ny, nz = 512, 512, 256
     grid = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
     ncell = nx * ny * nz
+    out = hip.malloc(nreal * ncell * 4)
+    stats = hip.malloc(nreal * 3 * 8)
+    # untuned: a fresh context (empty tuning table), same work
+    with S.Engine(0) as e0:
+        sp0 = e0.spectrum(sts, 0.0, 1.0, 1.0, 1.0, *grid)
+        try:
+            e0.realize_device(sp0, 4711, 1, 0, nreal, nx, ny, nz, S.FP32, out, stats)
+            e0.synchronize()
+            t0 = time.perf_counter()
+            e0.realize_device(sp0, 4711, 1, 0, nreal, nx, ny, nz, S.FP32, out, stats)
+            e0.synchronize()
+            t_untuned = time.perf_counter() - t0
+        finally:
+            sp0.close()
     t0 = time.perf_counter()
     spec = eng.spectrum(sts, 0.0, 1.0, 1.0, 1.0, *grid)
     eng.synchronize()
     spec_s = time.perf_counter() - t0
-    plan = eng.autotune(spec, nx, ny, nz, 1, reps=1)
-    out = hip.malloc(nreal * ncell * 4)
-    stats = hip.malloc(nreal * 3 * 8)
+    eng.autotune(spec, nx, ny, nz, 1, reps=1)
+    plan = tile_plan(eng, grid)
     try:
         eng.realize_device(spec, 4711, 1, 0, nreal, nx, ny, nz, S.FP32, out, stats)  # warm-up (same work)
         eng.synchronize()
@@ -441,6 +480,7 @@ def large_job(S, eng, hip, nreal=32, checks=(0, 31)):
                         f"ipad=1, igauss=1, {nreal} FP32 realizations in HBM",
             "periodic_grid": list(grid), "spectrum_s": round(spec_s, 3), "radix_plan": plan,
             "realizations_s": round(t_real, 4), "gcells_s": round(nreal * ncell / t_real / 1e9, 2),
+            "untuned_gcells_s": round(nreal * ncell / t_untuned / 1e9, 2),
             "fp32_vs_fp64_rel_rms": {str(r): e for r, e in zip(checks, errs)},
             "variogram": {"dirs": dirs, "seconds": round(t_var, 4),
                           "pairs_per_s": float(sum(S.npairs(nx, ny, nz, l * d[0], l * d[1], l * d[2])
@@ -583,6 +623,13 @@ def spectrum_file_leg(S, eng, spec, grid, build_ms):
             "build_on_device_ms": round(build_ms, 2)}
 
 
+def tile_plan(eng, grid):
+    """The tuned radix plan and tile of the three FP32 passes of a periodic grid."""
+    lx, ly, lz = (int(v).bit_length() - 1 for v in grid)
+    return {name: eng.tuning_get_tile(kind, L) for name, kind, L in
+            (("real_z_fp32", 6, lz), ("real_y_fp32", 7, ly), ("real_x_c2r_fp32", 8, max(lx - 1, 0)))}
+
+
 def load_traffic(kind):
     """HBM bytes per launch of a pass kind measured with rocprofv3 counters (profiles/traffic_c2.json:
     FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB device copy)."""
@@ -669,6 +716,20 @@ def main():
     spec_ms = (time.perf_counter() - t0) * 1e3
     nreal = args.nreal
     ncell = nx * ny * nz
+    untuned = None
+    if not args.plan:
+        # the step with the untuned table (largest radix 32, default tiles): 1 warm-up + 2 steps
+        o = hip.malloc(nreal * ncell * 4)
+        st0 = hip.malloc(nreal * 3 * 8)
+        for k in range(3):
+            if k == 1:
+                eng.synchronize()
+                t0 = time.perf_counter()
+            eng.realize_device(spec, ISEED, IGAUSS, k * nreal, nreal, nx, ny, nz, S.FP32, o, st0)
+        eng.synchronize()
+        untuned = 2 * nreal * ncell / (time.perf_counter() - t0) / 1e9
+        hip.free(o)
+        hip.free(st0)
     t0 = time.perf_counter()
     if args.plan:
         lx, ly, lz = (int(v).bit_length() - 1 for v in grid)
@@ -679,6 +740,7 @@ def main():
     else:
         plan = eng.autotune(spec, nx, ny, nz, IGAUSS, reps=3)
     tune_s = time.perf_counter() - t0
+    plans_tiles = tile_plan(eng, grid)
     out32 = hip.malloc(nreal * ncell * 4)
     stats = hip.malloc(nreal * 3 * 8)
 
@@ -762,7 +824,9 @@ def main():
                         "ms_per_realization": round(t64 / (args.fp64_steps * nreal) * 1e3, 4)} if t64 else None),
         "fp32_over_fp64": round(value / fp64_value, 2) if t64 else None,
         "spectrum_ms": round(spec_ms, 2),
-        "autotune": {"largest_radix_log2": plan, "seconds": round(tune_s, 2)},
+        "autotune": {"largest_radix_log2": plan, "plans_and_tiles": plans_tiles,
+                     "untuned_gcells_s": round(untuned, 3) if untuned else None, "tuned_gcells_s": round(value, 3),
+                     "seconds": round(tune_s, 2)},
         "uneven_batch": uneven,
         "large_job": large,
         "output_files": writer,

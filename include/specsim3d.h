@@ -58,8 +58,9 @@ typedef struct ss3d_context ss3d_context;
 typedef struct ss3d_spectrum ss3d_spectrum;
 
 /* ABI version, major * 10000 + minor * 100: 1.1 adds the output writer, spectrum files and
- * conditioning to 1.0 (grid sizing, spectra, realizations, variograms, request loop, batch graph). */
-#define SS3D_VERSION 10100
+ * conditioning to 1.0 (grid sizing, spectra, realizations, variograms, request loop, batch graph);
+ * 1.2 adds the allocation-free realization entry point and the tile entries of the tuning table. */
+#define SS3D_VERSION 10200
 int ss3d_version(void);
 
 /* Copies the last error message of the calling thread (NUL-terminated, truncated to len). */
@@ -155,13 +156,24 @@ int ss3d_context_profile(ss3d_context* ctx, int enable);
 int ss3d_context_profile_read(ss3d_context* ctx, double* ms, double* bytes, int64_t* launches, int nkinds);
 
 /* Autotuner of the FP32 passes (no counterpart in the reference): times every radix plan of
- * each FP32 pass (largest radix 8, 16 or 32) on this spectrum's grid and crop with `reps`
- * timed batches per plan, and keeps the fastest for later calls on the same context.
+ * each FP32 pass (largest radix 8, 16 or 32) and, for the strided passes, every tile variant
+ * (below) on this spectrum's grid and crop with `reps` timed batches per candidate, and keeps the
+ * fastest for later calls on the same context (the fill and y passes prefer a pair that has a
+ * fused launch).
  * tuning_get returns the plan (log2 of the largest radix) of pass kind 6..8 and length
  * 2^logn; tuning_set pins one (e.g. a plan stored from an earlier run). */
 int ss3d_autotune(ss3d_context* ctx, const ss3d_spectrum* spec, int nx, int ny, int nz, int igauss, int reps);
 int ss3d_tuning_get(const ss3d_context* ctx, int kind, int logn);
 int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
+/* The tile of a tuning entry (no counterpart in the reference).  The autotuner also picks a tile
+ * variant of the strided passes (kinds 6 and 7): variant 0 is the default tile, variant 1 halves
+ * the tile width for lengths up to 2^8 and drops the LDS position padding at length 2^10 (plans 3
+ * and 4 only).  The values computed do not depend on the tile.  get_tile returns the plan, the
+ * variant, the columns per tile (rows per block for kind 8) and whether the LDS exchange is padded;
+ * set_tile pins a plan and variant (SS3D_EINVAL where the variant does not exist).  Fused launches
+ * use variant 0. */
+int ss3d_tuning_get_tile(const ss3d_context* ctx, int kind, int logn, int* rm, int* variant, int* tile, int* lds_pad);
+int ss3d_tuning_set_tile(ss3d_context* ctx, int kind, int logn, int rm, int variant);
 
 /* Context options (no counterpart in the reference).
  * SS3D_OPT_PIPELINE (default 0): FP32 realizations run the fill pass of batch k+1 on a second
@@ -175,6 +187,9 @@ int ss3d_tuning_set(ss3d_context* ctx, int kind, int logn, int rm);
  * blocks interleave (arithmetic-bound and memory-bound tiles share each CU), where the passes' radix
  * plans have such a launch; 0 runs them as separate launches.  Same values. */
 #define SS3D_OPT_FUSE 3
+/* SS3D_OPT_BATCH_AUTOTUNE (default 1): ss3d_batch_create autotunes every distinct grid of its
+ * jobs; 0 keeps the context's tuning table as it is (for comparisons with untuned plans). */
+#define SS3D_OPT_BATCH_AUTOTUNE 4
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
 /* ---------------------------------------------------------------- request loop

@@ -23,6 +23,7 @@ FP32 = 1
 OPT_PIPELINE = 1
 OPT_FP64_GENERIC = 2
 OPT_FUSE = 3
+OPT_BATCH_AUTOTUNE = 4
 SCRATCH_BYTES = 256 << 20  # SS3D_SCRATCH_BYTES: the scratch array of ss3d_realize_device_noalloc
 
 ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device", 6: "I/O error"}
@@ -128,6 +129,8 @@ SIGNATURES = {
     "ss3d_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ss3d_tuning_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ss3d_tuning_set": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ss3d_tuning_get_tile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip]),
+    "ss3d_tuning_set_tile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "ss3d_context_set_option": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ss3d_server_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)]),
     "ss3d_server_submit": (ctypes.c_int, [_vp, ctypes.c_void_p]),
@@ -371,6 +374,15 @@ class Engine:
 
     def tuning_set(self, kind, logn, rm):
         check(lib().ss3d_tuning_set(self.handle, kind, logn, rm))
+
+    def tuning_get_tile(self, kind, logn):
+        """dict(rm, variant, tile, lds_pad) of a tuning entry (ss3d_tuning_get_tile)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().ss3d_tuning_get_tile(self.handle, kind, logn, *[ctypes.byref(x) for x in v]))
+        return dict(rm=v[0].value, variant=v[1].value, tile=v[2].value, lds_pad=bool(v[3].value))
+
+    def tuning_set_tile(self, kind, logn, rm, variant):
+        check(lib().ss3d_tuning_set_tile(self.handle, kind, logn, rm, variant))
 
     def tuning_get(self, kind, logn):
         """Radix plan (log2 of the largest radix) of FP32 pass kind 6..8 at length 2^logn."""

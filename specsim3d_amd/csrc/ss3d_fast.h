@@ -283,12 +283,15 @@ __device__ __forceinline__ int out_pos(int b, int r) {
 // Half-spectrum column 0 carries the Nyquist column kx = npx/2 in its imaginary part:
 // c'(0, ky, kz) = c(0, ky, kz) + i c(npx/2, ky, kz).  Both planes are Hermitian in (ky, kz),
 // so after the y and z transforms column 0 holds g0 + i gN with g0, gN real.
-template <int L, int P, int RM>
+// PAD: one padding slot after every EPT positions in the exchange buffer (by default for tiles
+// narrower than 16 columns, whose 16-byte lanes would otherwise meet in few LDS banks); a tile
+// variant of the autotuner flips it.
+template <int L, int P, int RM, bool PAD = (P < 16)>
 struct StridedCfg {
     static_assert(P % 2 == 0, "column pairs");
     static constexpr int N = 1 << L, EL = eptlog(L, RM), EPT = 1 << EL, G = N / EPT, NT = G * (P / 2);
     static constexpr int M = npass(L, RM);
-    static constexpr bool PADPOS = P < 16;
+    static constexpr bool PADPOS = PAD;
     static constexpr int ppos(int pos) { return PADPOS ? pos + (pos >> EL) : pos; }
     static constexpr size_t lds_elems() { return (size_t)(N + (PADPOS ? (N >> EL) : 0)) * P; }
     // lengths up to 2^11: the W_n table (n entries) is copied to LDS behind the exchange buffer,
@@ -333,9 +336,9 @@ __device__ __forceinline__ void st_nt16(float4 v, float4* p) {
 // HALF (passes that keep at most half the axis, n_out <= N/2): the last radix pass computes
 // only the outputs r < R/2 of each butterfly.  Output r of a last-pass butterfly sits at position
 // pos0 + r N/R with pos0 < N/R, so the outputs r >= R/2 lie at N/2 or beyond and are never stored.
-template <int L, int P, int IN, int AXIS, int RM, int IG, bool HALF = false>
+template <int L, int P, int IN, int AXIS, int RM, int IG, bool HALF = false, bool PAD = (P < 16)>
 __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, const int grp) {
-    using C = StridedCfg<L, P, RM>;
+    using C = StridedCfg<L, P, RM, PAD>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, PH = P / 2;
     static_assert(M > 0, "strided passes have length >= 2");
     float4* lds4 = reinterpret_cast<float4*>(fast_smem);
@@ -576,9 +579,9 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
     }
 }
 
-template <int L, int P, int IN, int AXIS, int RM, int IG>
-__global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void strided_kernel(PassArgs a) {
-    strided_tile<L, P, IN, AXIS, RM, IG>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.y);
+template <int L, int P, int IN, int AXIS, int RM, int IG, bool PAD = (P < 16)>
+__global__ __launch_bounds__((StridedCfg<L, P, RM, PAD>::NT)) void strided_kernel(PassArgs a) {
+    strided_tile<L, P, IN, AXIS, RM, IG, false, PAD>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.y);
 }
 
 // ------------------------------------------------------------ fill pass beside a y pass

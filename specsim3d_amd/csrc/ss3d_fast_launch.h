@@ -9,6 +9,15 @@ namespace fastlaunch {
 
 constexpr int strided_P(int L) { return L <= 7 ? 64 : (L >= 12 ? 2 : (8192 >> L)); }
 
+// Tile variants of the strided passes, chosen per length and pass by the autotuner (results are
+// the same bit for bit: each column's transform does not depend on the tile).  Variant 0 is the
+// tile above; variant 1 halves the tile width for lengths up to 2^8 (twice the blocks for small
+// grids) and drops the LDS position padding for length 2^10 (tiles of 8 columns).  Variant 1 is
+// instantiated for radix plans 3 and 4; fused launches use variant 0.
+constexpr bool tile_variant_exists(int L, int RM) { return (L <= 8 || L == 10) && RM <= 4; }
+constexpr int tile_P(int L, int V) { return V == 1 && L <= 8 ? strided_P(L) / 2 : strided_P(L); }
+constexpr bool tile_pad(int L, int V) { return V == 1 && L == 10 ? false : tile_P(L, V) < 16; }
+
 template <typename K>
 hipError_t launch_k(K k, size_t lds, int nt, int nblocks, int nbatch, const PassArgs& a, hipStream_t s, bool& attr) {
     if (lds > 65536 && !attr) {
@@ -20,12 +29,14 @@ hipError_t launch_k(K k, size_t lds, int nt, int nblocks, int nbatch, const Pass
     return hipGetLastError();
 }
 
-template <int L, int IN, int AXIS, int RM, int IG>
+template <int L, int IN, int AXIS, int RM, int IG, int V = 0>
 hipError_t strided_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
-    using C = fast::StridedCfg<L, strided_P(L), RM>;
+    constexpr int P = tile_P(L, V);
+    constexpr bool PAD = tile_pad(L, V);
+    using C = fast::StridedCfg<L, P, RM, PAD>;
     static bool attr = false;
-    return launch_k(fast::strided_kernel<L, strided_P(L), IN, AXIS, RM, IG>, C::lds_bytes(), C::NT,
-                    nblocks, nbatch, a, s, attr);
+    return launch_k(fast::strided_kernel<L, P, IN, AXIS, RM, IG, PAD>, C::lds_bytes(), C::NT, nblocks, nbatch, a, s,
+                    attr);
 }
 
 // y lengths of a fused launch beside a fill of length 2^L and plan RM: 2^L (plan RM) or, with
@@ -111,11 +122,20 @@ hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     return launch_k(fast::c2r_kernel<L, IN, RM>, C::lds_elems() * sizeof(float2), C::NT, nblocks, nbatch, a, s, attr);
 }
 
+template <int L, int IN, int AXIS, int RM, int IG>
+hipError_t strided_LV(int var, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    if (var == 0) return strided_L<L, IN, AXIS, RM, IG, 0>(nblocks, nbatch, a, s);
+    if constexpr (tile_variant_exists(L, RM)) {
+        if (var == 1) return strided_L<L, IN, AXIS, RM, IG, 1>(nblocks, nbatch, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <int RM, int IN, int AXIS, int IG, int... Ls>
-hipError_t strided_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
+hipError_t strided_dispatch(int L, int var, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
                             std::integer_sequence<int, Ls...>) {
     hipError_t e = hipErrorInvalidValue;
-    ((L == Ls + 1 ? (e = strided_L<Ls + 1, IN, AXIS, RM, IG>(nblocks, nbatch, a, s), 0) : 0), ...);
+    ((L == Ls + 1 ? (e = strided_LV<Ls + 1, IN, AXIS, RM, IG>(var, nblocks, nbatch, a, s), 0) : 0), ...);
     return e;
 }
 
@@ -128,16 +148,17 @@ hipError_t c2r_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipSt
 }
 
 template <int RM>
-hipError_t strided_rm(int L, int in_mode, int axis, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+hipError_t strided_rm(int L, int in_mode, int axis, int var, int nblocks, int nbatch, const PassArgs& a,
+                      hipStream_t s) {
     using seq = std::make_integer_sequence<int, kMaxFastLog>;
     // fill passes: one instantiation per coefficient law (igauss 0 or 1)
-    if (in_mode == IN_LOAD) return strided_dispatch<RM, IN_LOAD, 1, 0>(L, nblocks, nbatch, a, s, seq{});
+    if (in_mode == IN_LOAD) return strided_dispatch<RM, IN_LOAD, 1, 0>(L, var, nblocks, nbatch, a, s, seq{});
     if (in_mode == IN_FILL32H && axis == 2)
-        return a.igauss ? strided_dispatch<RM, IN_FILL32H, 2, 1>(L, nblocks, nbatch, a, s, seq{})
-                        : strided_dispatch<RM, IN_FILL32H, 2, 0>(L, nblocks, nbatch, a, s, seq{});
+        return a.igauss ? strided_dispatch<RM, IN_FILL32H, 2, 1>(L, var, nblocks, nbatch, a, s, seq{})
+                        : strided_dispatch<RM, IN_FILL32H, 2, 0>(L, var, nblocks, nbatch, a, s, seq{});
     if (in_mode == IN_FILL32H && axis == 1)
-        return a.igauss ? strided_dispatch<RM, IN_FILL32H, 1, 1>(L, nblocks, nbatch, a, s, seq{})
-                        : strided_dispatch<RM, IN_FILL32H, 1, 0>(L, nblocks, nbatch, a, s, seq{});
+        return a.igauss ? strided_dispatch<RM, IN_FILL32H, 1, 1>(L, var, nblocks, nbatch, a, s, seq{})
+                        : strided_dispatch<RM, IN_FILL32H, 1, 0>(L, var, nblocks, nbatch, a, s, seq{});
     return hipErrorInvalidValue;
 }
 

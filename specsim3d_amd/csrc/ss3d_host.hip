@@ -322,6 +322,7 @@ struct ss3d_context {
     Lane sl;
     bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
     bool fuse = true;        // SS3D_OPT_FUSE: FP32 fill of batch k+1 and y pass of batch k in one launch
+    bool batch_tune = true;  // SS3D_OPT_BATCH_AUTOTUNE: batch creation autotunes every distinct grid
     bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
@@ -500,6 +501,7 @@ struct Pass {
                    // FP64 compile-time kernels: 4 x pass (fill + rows), 5 strided,
                    // 6 x pass of the spectral density (covariance rows)
     int rm = 5;    // FP32 fast kernels: largest radix 2^rm
+    int var = 0;   // FP32 strided kernels: tile variant (fast_strided_tile / fast_strided_pad)
 };
 
 // Bytes a pass must move through HBM per batch member: the operands it reads (complex input,
@@ -549,7 +551,7 @@ hipEvent_t take_event(ss3d_context* c) {
 int prep_pass(Pass& p, int nbatch, hipStream_t strm, DevBuf& partial, int& groups) {
     p.a.logp = p.sh.logp;
     int P = 1 << p.sh.logp;
-    if (p.fast == 1) P = fast_strided_tile(p.a.logn);
+    if (p.fast == 1) P = fast_strided_tile(p.a.logn, p.var);
     if (p.fast == 2) P = fast_c2r_rows(p.a.logn, p.rm);
     if (p.fast == 3) P = 256;
     if (p.fast >= 4) P = fast64_tile(p.fast, p.a.logn);
@@ -594,7 +596,7 @@ int run_pass(ss3d_context* c, Pass& p, int nbatch, hipStream_t strm, DevBuf& par
     }
     hipError_t e;
     if (p.fast == 1)
-        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.nblocks, groups, p.a, strm);
+        e = launch_fast_strided(p.a.logn, p.in, p.a.axis, p.rm, p.var, p.nblocks, groups, p.a, strm);
     else if (p.fast == 2)
         e = launch_fast_c2r(p.a.logn, p.in, p.rm, p.nblocks, nbatch, p.a, strm);
     else if (p.fast == 3)
@@ -974,9 +976,22 @@ namespace {
 
 int batch_size(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal);
 
+// The tuning table holds rm + 16 var: the radix plan and the tile variant of a pass kind and length.
 int tuned_rm(const ss3d_context* c, int kind, int L) {
     auto it = c->tune.find({kind, L});
-    return it == c->tune.end() ? 5 : it->second;
+    return it == c->tune.end() ? 5 : it->second & 15;
+}
+int tuned_var(const ss3d_context* c, int kind, int L) {
+    auto it = c->tune.find({kind, L});
+    return it == c->tune.end() ? 0 : it->second >> 4;
+}
+
+// Columns per tile of the FP32 fill output when it is laid out in column tiles (the z and y passes
+// have one length and one tile variant, so a y tile reads one contiguous slab), else 0.
+int fill_column_tile(const ss3d_context* c, const ss3d_spectrum* sp) {
+    const int ly = ilog2(sp->npy), lz = ilog2(sp->npz);
+    if (sp->npy < 2 || sp->npz < 2 || ly != lz || tuned_var(c, 6, lz) != tuned_var(c, 7, ly)) return 0;
+    return fast_strided_tile(lz, tuned_var(c, 6, lz));
 }
 
 // Enqueue realizations r0..r0+nb-1 into dout (device) and their statistics into dstats.
@@ -1101,15 +1116,14 @@ struct Fp32Work {
     int64_t fill, yout;
     bool staged, has_wb;
 };
-Fp32Work fp32_work(const ss3d_spectrum* sp, int nx, int ny, int nz) {
+Fp32Work fp32_work(const ss3d_spectrum* sp, int nx, int ny, int nz, int ctile) {
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const bool has_y = npy > 1, has_z = npz > 1;
     const int hc = npx >= 2 ? npx / 2 : 1, hcs = std::max(2, hc);
     const int64_t bz = (int64_t)hcs * npy * nz, by = (int64_t)hcs * ny * nz;
     int64_t bzs = bz;
-    if (has_z && has_y && ilog2(npz) == ilog2(npy)) {
-        const int P = fast_strided_tile(ilog2(npz));
-        const int64_t slab = (int64_t)npy * P, plane = (int64_t)((hc + P - 1) / P) * slab;
+    if (ctile > 0) {
+        const int64_t slab = (int64_t)npy * ctile, plane = (int64_t)((hc + ctile - 1) / ctile) * slab;
         bzs = plane * nz;
     }
     Fp32Work w;
@@ -1156,6 +1170,7 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     // pass templates; `in`/`out` buffers and rbase are set per batch
     Pass pz{32, IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 6, 1};
     pz.rm = tuned_rm(c, 6, lz);
+    pz.var = tuned_var(c, 6, lz);
     pz.a.axis = 2;
     pz.a.logn = lz;
     pz.a.n0 = hc;
@@ -1167,6 +1182,7 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     pz.a.tw = wtab(lz);
     Pass py{32, has_z ? IN_LOAD : IN_FILL32H, OUT_STORE, Shape{0, 0}, 0, a, 7, 1};
     py.rm = tuned_rm(c, 7, ly);
+    py.var = tuned_var(c, 7, ly);
     py.a.axis = 1;
     py.a.logn = ly;
     py.a.n0 = hc;
@@ -1179,11 +1195,11 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     py.a.out_sT = (int64_t)hcs * ny;
     py.a.out_bstride = by;
     py.a.tw = wtab(ly);
-    // z and y of one length (one tile width P): the fill writes BZ in column tiles, a slab of
-    // P columns x npy rows per tile and z plane, so that a y tile reads one contiguous slab
+    // z and y of one length and tile variant (one tile width P): the fill writes BZ in column
+    // tiles, a slab of P columns x npy rows per tile and z plane, so that a y tile reads one
+    // contiguous slab
     int64_t bzs = bz;
-    if (has_z && has_y && lz == ly) {
-        const int P = fast_strided_tile(lz);
+    if (const int P = fill_column_tile(c, sp)) {
         const int64_t slab = (int64_t)npy * P, plane = (int64_t)((hc + P - 1) / P) * slab;
         bzs = plane * nz;
         pz.a.out_sA = plane;
@@ -1209,7 +1225,7 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     px.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
 
     const bool staged = has_y || has_z;  // a fill pass followed by further passes
-    const Fp32Work wk = fp32_work(sp, nx, ny, nz);
+    const Fp32Work wk = fp32_work(sp, nx, ny, nz, fill_column_tile(c, sp));
     hipStream_t s2 = c->pipeline ? L.stream2 : L.stream;  // stream of the passes after the fill
     const bool pipelined = staged && c->pipeline;
     if (staged) {
@@ -1220,8 +1236,8 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     // fusion keeps the y pass's own radix plan (results do not depend on the schedule): the fill's
     // plan at the same length, plan 3 at twice the length
     const int rmy_fused = lz == ly ? pz.rm : 3;
-    if (staged && has_z && has_y && !pipelined && c->fuse && nbat >= 2 && fast_same_plan(ly, py.rm, rmy_fused) &&
-        fast_fusable(lz, pz.rm, ly, rmy_fused)) {
+    if (staged && has_z && has_y && !pipelined && c->fuse && nbat >= 2 && pz.var == 0 && py.var == 0 &&
+        fast_same_plan(ly, py.rm, rmy_fused) && fast_fusable(lz, pz.rm, ly, rmy_fused)) {
         // launch k: fill of batch k beside the y pass of batch k-1 (pipe buffers alternate),
         // then the x pass of batch k-1
         py.rm = rmy_fused;  // the same plan, the template instance of the fused kernel
@@ -1375,11 +1391,12 @@ struct ScratchPlan {
     size_t rbase = 0, stats = 0, partial = 0, w1 = 0, w2 = 0, w3 = 0;
 };
 
-int plan_scratch(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal, bool own_stats, ScratchPlan& q) {
+int plan_scratch(const ss3d_context* c, const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nreal,
+                 bool own_stats, ScratchPlan& q) {
     const int64_t npy = sp->npy, npz = sp->npz;
     int64_t blocks, per1, per2 = 0, per3 = 0;  // blocks with statistics, buffer bytes per member
     if (prec == SS3D_FP32) {
-        const Fp32Work w = fp32_work(sp, nx, ny, nz);
+        const Fp32Work w = fp32_work(sp, nx, ny, nz, fill_column_tile(c, sp));
         blocks = (int64_t)ny * nz;  // x pass: at most one block per row
         if (w.staged) {
             per1 = per2 = w.fill * 8;
@@ -1430,7 +1447,7 @@ int ss3d_realize_device_noalloc(ss3d_context* c, const ss3d_spectrum* spc, int i
                     "one ss3d_realize or ss3d_realize_device call with this igauss builds them",
                     igauss);
     ScratchPlan q;
-    SS3D_TRY(plan_scratch(sp, nx, ny, nz, prec, nreal, dev_stats == nullptr, q));
+    SS3D_TRY(plan_scratch(c, sp, nx, ny, nz, prec, nreal, dev_stats == nullptr, q));
     HIP_TRY(hipSetDevice(c->device));
     // windows of the scratch array; buffers this call does not use get none
     Lane& L = c->sl;
@@ -1533,11 +1550,20 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
     SS3D_TRY(ensure2(c->main, c->main.dout, (size_t)nb * ncrop * 4));
     SS3D_TRY(ensure2(c->main, c->main.stats, (size_t)nb * 3 * sizeof(double)));
     const bool was = c->profiling;
-    double best[3] = {1e30, 1e30, 1e30}, tz[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
-    double ty[6] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
-    int pick[3] = {5, 5, 5};
-    for (int rm = 3; rm <= 5; ++rm) {
-        for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = rm;
+    // candidates: radix plans 3..5 with the default tile, plans 3..4 with tile variant 1 (where the
+    // strided lengths have it); entry = rm + 16 var, as in the tuning table
+    const int cand[5] = {3, 4, 5, 3 + 16, 4 + 16};
+    std::map<int, double> t[3];
+    for (int e : cand) {
+        const int rm = e & 15, var = e >> 4;
+        bool any = var == 0;
+        for (int k = 0; k < 2; ++k) {
+            const bool v = var == 1 && fast_tile_variant_exists(Lk[k], rm);
+            any = any || v;
+            c->tune[{6 + k, Lk[k]}] = rm + (v ? 16 : 0);
+        }
+        if (!any) continue;
+        c->tune[{8, Lk[2]}] = rm;
         SS3D_TRY(enqueue_fp32(c, c->main, sp, 1, igauss, 0, nb, nx, ny, nz, c->main.dout.p, c->main.stats.as<double>()));
         SS3D_TRY(ss3d_context_profile(c, 1));
         for (int r = 0; r < std::max(1, reps); ++r)
@@ -1545,27 +1571,34 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
         double ms[SS3D_NPASS_KINDS], by[SS3D_NPASS_KINDS];
         int64_t nl[SS3D_NPASS_KINDS];
         SS3D_TRY(ss3d_context_profile_read(c, ms, by, nl, SS3D_NPASS_KINDS));
-        for (int k = 0; k < 3; ++k)
-            if (nl[6 + k] && ms[6 + k] < best[k]) {
-                best[k] = ms[6 + k];
-                pick[k] = rm;
-            }
-        if (nl[6]) tz[rm] = ms[6];
-        if (nl[7]) ty[rm] = ms[7];
+        for (int k = 0; k < 3; ++k) {
+            const int got = c->tune[{6 + k, Lk[k]}];
+            if (nl[6 + k] && (k < 2 || var == 0) && !t[k].count(got)) t[k][got] = ms[6 + k];
+        }
     }
     SS3D_TRY(ss3d_context_profile(c, was ? 1 : 0));
-    // The fill and the y pass run fused only with plans that have a fused kernel (one plan at one
-    // length, plan 3 for a y pass twice as long): where one exists, take the pair with the
-    // smallest sum of their times.
-    if (sp->npy > 1 && sp->npz > 1 && !fast_fusable(lz, pick[0], ly, pick[1])) {
+    int pick[3] = {5, 5, 5};
+    for (int k = 0; k < 3; ++k) {
+        double b = 1e30;
+        for (auto& kv : t[k])
+            if (kv.second < b) {
+                b = kv.second;
+                pick[k] = kv.first;
+            }
+    }
+    // The fill and the y pass run fused only with the default tiles and plans that have a fused
+    // kernel (one plan at one length, plan 3 for a y pass twice as long): where one exists, take
+    // the pair with the smallest sum of their times.
+    auto fusable = [&](int ez, int ey) { return ez < 16 && ey < 16 && fast_fusable(lz, ez, ly, ey); };
+    if (sp->npy > 1 && sp->npz > 1 && !fusable(pick[0], pick[1])) {
         double bestsum = 1e30;
         int pz = 0, py = 0;
-        for (int a = 3; a <= 5; ++a)
-            for (int b = 3; b <= 5; ++b)
-                if (fast_fusable(lz, a, ly, b) && tz[a] + ty[b] < bestsum) {
-                    bestsum = tz[a] + ty[b];
-                    pz = a;
-                    py = b;
+        for (auto& a : t[0])
+            for (auto& b : t[1])
+                if (fusable(a.first, b.first) && a.second + b.second < bestsum) {
+                    bestsum = a.second + b.second;
+                    pz = a.first;
+                    py = b.first;
                 }
         if (pz) {
             pick[0] = pz;
@@ -1592,6 +1625,10 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
         c->fuse = value != 0;
         return SS3D_OK;
     }
+    if (option == SS3D_OPT_BATCH_AUTOTUNE) {
+        c->batch_tune = value != 0;
+        return SS3D_OK;
+    }
     return fail(SS3D_EINVAL, "unknown option %d", option);
 }
 
@@ -1600,11 +1637,27 @@ int ss3d_tuning_get(const ss3d_context* c, int kind, int logn) {
     return tuned_rm(c, kind, logn);
 }
 
-int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) {
+int ss3d_tuning_set(ss3d_context* c, int kind, int logn, int rm) { return ss3d_tuning_set_tile(c, kind, logn, rm, 0); }
+
+int ss3d_tuning_get_tile(const ss3d_context* c, int kind, int logn, int* rm, int* variant, int* tile, int* lds_pad) {
+    if (!c) return fail(SS3D_EINVAL, "null context");
+    if (kind < 6 || kind > 8 || logn < 0 || logn > kMaxFastLog)
+        return fail(SS3D_EINVAL, "tuning entry out of range (kind 6..8, logn 0..%d)", kMaxFastLog);
+    const int r = tuned_rm(c, kind, logn), v = tuned_var(c, kind, logn);
+    if (rm) *rm = r;
+    if (variant) *variant = v;
+    if (tile) *tile = kind == 8 ? fast_c2r_rows(logn, r) : fast_strided_tile(logn, v);
+    if (lds_pad) *lds_pad = kind == 8 ? 1 : (fast_strided_pad(logn, v) ? 1 : 0);
+    return SS3D_OK;
+}
+
+int ss3d_tuning_set_tile(ss3d_context* c, int kind, int logn, int rm, int variant) {
     if (!c) return fail(SS3D_EINVAL, "null context");
     if (kind < 6 || kind > 8 || logn < 0 || logn > kMaxFastLog || rm < 3 || rm > 5)
         return fail(SS3D_EINVAL, "tuning entry out of range (kind 6..8, logn 0..%d, rm 3..5)", kMaxFastLog);
-    c->tune[{kind, logn}] = rm;
+    if (variant != 0 && (variant != 1 || kind == 8 || !fast_tile_variant_exists(logn, rm)))
+        return fail(SS3D_EINVAL, "no tile variant %d for pass kind %d, length 2^%d, plan %d", variant, kind, logn, rm);
+    c->tune[{kind, logn}] = rm + 16 * variant;
     return SS3D_OK;
 }
 

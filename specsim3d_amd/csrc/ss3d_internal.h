@@ -96,10 +96,12 @@ hipError_t launch_diag_finalize(const double* partial, int nblocks, double* diag
 // its length (entry m = exp(+2 pi i m/n), FP32); a C2R pass gets the W_N table of its complex
 // length N = npx/2, immediately followed by the W_2N table.
 // rm: largest radix 2^rm of the pass plan (3, 4 or 5).
-int fast_strided_tile(int L);        // columns per block
+int fast_strided_tile(int L, int var = 0);     // columns per block of tile variant var
+bool fast_strided_pad(int L, int var = 0);     // LDS position padding of tile variant var
+bool fast_tile_variant_exists(int L, int rm);  // variant 1 instantiated for this length and plan
 int fast_c2r_rows(int L, int rm);    // rows per block
-hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int nblocks, int nbatch, const PassArgs& a,
-                               hipStream_t s);
+hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int var, int nblocks, int nbatch,
+                               const PassArgs& a, hipStream_t s);
 hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 hipError_t launch_fast_x1(int nblocks, int nbatch, const PassArgs& a, hipStream_t s);
 // z fill pass (az) beside the y pass (ay) of the previous batch in one launch of

@@ -3,9 +3,9 @@
 
 namespace ss3d {
 
-hipError_t launch_fast_strided_r3(int, int, int, int, int, const PassArgs&, hipStream_t);
-hipError_t launch_fast_strided_r4(int, int, int, int, int, const PassArgs&, hipStream_t);
-hipError_t launch_fast_strided_r5(int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_strided_r3(int, int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_strided_r4(int, int, int, int, int, int, const PassArgs&, hipStream_t);
+hipError_t launch_fast_strided_r5(int, int, int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_fused_r3(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
 hipError_t launch_fast_fused_r4(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
 hipError_t launch_fast_fused_r5(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
@@ -13,7 +13,9 @@ hipError_t launch_fast_c2r_r3(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r4(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r5(int, int, int, int, const PassArgs&, hipStream_t);
 
-int fast_strided_tile(int L) { return fastlaunch::strided_P(L); }
+int fast_strided_tile(int L, int var) { return fastlaunch::tile_P(L, var); }
+bool fast_strided_pad(int L, int var) { return fastlaunch::tile_pad(L, var); }
+bool fast_tile_variant_exists(int L, int rm) { return fastlaunch::tile_variant_exists(L, rm); }
 
 int fast_c2r_rows(int L, int rm) {
     const int el = rm == 3 ? fast::eptlog(L, 3) : rm == 4 ? fast::eptlog(L, 4) : fast::eptlog(L, 5);
@@ -21,13 +23,13 @@ int fast_c2r_rows(int L, int rm) {
     return g >= 256 ? 1 : 256 / g;
 }
 
-hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int nblocks, int nbatch, const PassArgs& a,
-                               hipStream_t s) {
+hipError_t launch_fast_strided(int L, int in_mode, int axis, int rm, int var, int nblocks, int nbatch,
+                               const PassArgs& a, hipStream_t s) {
     if (L < 1 || L > kMaxFastLog) return hipErrorInvalidValue;
     switch (rm) {
-        case 3: return launch_fast_strided_r3(L, in_mode, axis, nblocks, nbatch, a, s);
-        case 4: return launch_fast_strided_r4(L, in_mode, axis, nblocks, nbatch, a, s);
-        case 5: return launch_fast_strided_r5(L, in_mode, axis, nblocks, nbatch, a, s);
+        case 3: return launch_fast_strided_r3(L, in_mode, axis, var, nblocks, nbatch, a, s);
+        case 4: return launch_fast_strided_r4(L, in_mode, axis, var, nblocks, nbatch, a, s);
+        case 5: return launch_fast_strided_r5(L, in_mode, axis, var, nblocks, nbatch, a, s);
         default: return hipErrorInvalidValue;
     }
 }

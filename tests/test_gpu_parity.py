@@ -215,3 +215,35 @@ def test_fp64_compile_time_kernels_equal_generic(eng, grid):
     # statistics: the same sums over differently shaped blocks
     assert np.allclose(m1, m2, rtol=0, atol=1e-14) and np.allclose(v1, v2, rtol=1e-13)
     assert np.allclose(i1, i2, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("rm", [3, 4])
+@pytest.mark.parametrize("grid", [(64, 48, 20), (60, 50, 50), (100, 30, 1), (40, 600, 20), (20, 12, 700)])
+def test_tile_variants_give_identical_fields(eng, rm, grid):
+    """Every tile variant the autotuner may pick (halved tile width up to 2^8, no LDS padding at
+    2^10) computes the default tile's fields bit for bit: a column's transform does not depend on
+    the tile.  Covers column-tiled fill output (z and y of one length), 2D grids (fill along y) and
+    axes of 2^10."""
+    sts = [(2, 0.7, 8, 4, 2, 30, 15, 0)]
+    npg = S.periodic_grid(sts, 1, 1, 1, *grid, 1)
+    lx, ly, lz = (v.bit_length() - 1 for v in npg)
+    say(f"[tiles] rm {rm} grid {grid} periodic {npg}")
+    sp = eng.spectrum(sts, 0.3, 1, 1, 1, *npg)
+    try:
+        out = {}
+        for var in (0, 1):
+            for kind, L in ((6, lz), (7, ly)):
+                try:
+                    eng.tuning_set_tile(kind, L, rm, var)
+                except S.SS3DError:  # no variant at this length: default tile
+                    eng.tuning_set_tile(kind, L, rm, 0)
+            eng.tuning_set(8, max(lx - 1, 0), rm)
+            out[var] = eng.realize(sp, 4711, 1, 0, 3, *grid, S.FP32)
+            picked = [eng.tuning_get_tile(k, L) for k, L in ((6, lz), (7, ly))]
+            say(f"  variant {var}: {picked}")
+    finally:
+        for kind, L in ((6, lz), (7, ly), (8, max(lx - 1, 0))):
+            eng.tuning_set(kind, L, 5)
+        sp.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
