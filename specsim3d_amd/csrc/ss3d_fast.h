@@ -615,26 +615,27 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(P
 // ------------------------------------------------------------ C2R pass along x
 // Rows of the half spectrum X[0..n/2] (n = npx = 2^(L+1)) -> real row of n, cropped to nx,
 // with FP64 statistics.  Thread tid: row q = tid / G, butterfly slot g = tid % G.
-template <int L, int RM>
+// PR: rows per block (0: as many as make 256 threads; the fused launch asks for its tiles' count)
+template <int L, int RM, int PR = 0>
 struct C2RCfg {
     static constexpr int N = 1 << L, EL = eptlog(L, RM), EPT = 1 << EL, G = N / EPT;
-    static constexpr int P = G >= 256 ? 1 : 256 / G, NT = G * P, M = npass(L, RM);
+    static constexpr int P = PR ? PR : (G >= 256 ? 1 : 256 / G), NT = G * P, M = npass(L, RM);
     static constexpr int ROW = N + (N >> EL);
     static constexpr int ppos(int pos) { return pos + (pos >> EL); }
     static constexpr size_t lds_elems() { return (size_t)ROW * P; }
 };
 
-template <int L, int IN, int RM>
-__global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
-    using C = C2RCfg<L, RM>;
+// One block of the C2R pass: rows rblock*P .. +P-1 of batch member b.
+template <int L, int IN, int RM, int PR = 0>
+__device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, const int b) {
+    using C = C2RCfg<L, RM, PR>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, P = C::P, M = C::M;
     f2* lds = reinterpret_cast<f2*>(fast_smem);
     const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);    // W_N table
     const f2* __restrict__ tw2 = tw + N;                               // W_{2N} table (next length)
     const int tid = threadIdx.x;
     const int g = tid % G, q = tid / G;
-    const int row = blockIdx.x * P + q;
-    const int b = blockIdx.y;
+    const int row = rblock * P + q;
     const bool active = row < a.nrows;
     f2 v[EPT];
 
@@ -642,7 +643,7 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
         f2 xa[EPT], xb[EPT];
         if constexpr (IN == IN_LOADH) {
             const f2* __restrict__ src = reinterpret_cast<const f2*>(a.in) + (int64_t)b * a.in_bstride +
-                                         (int64_t)blockIdx.x * P * a.in_row;
+                                         (int64_t)rblock * P * a.in_row;
             const uint32_t base = (uint32_t)q * (uint32_t)a.in_row;
 #pragma unroll
             for (int r = 0; r < EPT; ++r) {  // read once: non-temporal
@@ -724,7 +725,7 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
     double s1 = 0.0, s2 = 0.0;
     if (active) {
         float* __restrict__ dst = reinterpret_cast<float*>(a.out) + (int64_t)b * a.out_bstride +
-                                  (int64_t)blockIdx.x * P * a.out_row;
+                                  (int64_t)rblock * P * a.out_row;
         const uint32_t base = (uint32_t)q * (uint32_t)a.out_row;
         constexpr int LRl = M ? rlog(L, M > 0 ? M - 1 : 0, RM) : 0, NBl = EPT >> LRl;
         const int nx = a.n_out;
@@ -755,27 +756,76 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
             }
     }
     // block statistics (sum, sum of squares, max|Im| = 0 for a real-output transform)
+    // (in the exchange buffer, once every wave is past its last LDS read: no static LDS, so the
+    // fused launch keeps two blocks per CU).  A block of PR rows writes the partials of the
+    // standard blocks it spans (their waves, in their order), so that the statistics do not depend
+    // on the launch that ran the pass; a.nblocks counts standard blocks.
     constexpr int NW = (C::NT + 63) / 64;
-    __shared__ double red[2][NW];
+    constexpr int SW = (C2RCfg<L, RM>::NT + 63) / 64, NSUB = NW / SW;
+    static_assert(NW % SW == 0 && C::P % C2RCfg<L, RM>::P == 0, "whole standard blocks");
+    static_assert(C::lds_elems() * sizeof(f2) >= 2 * NW * sizeof(double), "room for the block sums");
     s1 = wsum(s1);
     s2 = wsum(s2);
     const int lane = tid & 63, wid = tid >> 6;
+    double* red = reinterpret_cast<double*>(fast_smem);
+    __syncthreads();
     if (lane == 0) {
-        red[0][wid] = s1;
-        red[1][wid] = s2;
+        red[wid] = s1;
+        red[NW + wid] = s2;
     }
     __syncthreads();
     if (tid == 0) {
-        double t1 = 0, t2 = 0;
-        for (int w = 0; w < NW; ++w) {
-            t1 += red[0][w];
-            t2 += red[1][w];
+        for (int sb = 0; sb < NSUB; ++sb) {
+            const int blk = rblock * NSUB + sb;
+            if (blk >= a.nblocks) break;
+            double t1 = 0, t2 = 0;
+            for (int w = sb * SW; w < (sb + 1) * SW; ++w) {
+                t1 += red[w];
+                t2 += red[NW + w];
+            }
+            double* p = a.partial + 3 * ((int64_t)b * a.nblocks + blk);
+            p[0] = t1;
+            p[1] = t2;
+            p[2] = 0.0;
         }
-        double* p = a.partial + 3 * ((int64_t)b * a.nblocks + blockIdx.x);
-        p[0] = t1;
-        p[1] = t2;
-        p[2] = 0.0;
     }
+}
+
+template <int L, int IN, int RM>
+__global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
+    c2r_tile<L, IN, RM>(a, blockIdx.x, blockIdx.y);
+}
+
+// ------------------------------------------------------------ fill, y and C2R passes in one launch
+// The fill of batch k, the y pass of batch k-1 and the C2R x pass of batch k-2 (the x role reads
+// the y output of two launches before, which is double-buffered).  Roles interleave in periods
+// of 8A fill, 8B y and 8C x blocks (A:B:C the ratio of their block counts), K times, then the
+// rest of each.  The x role is memory-bound and light on VALU, so it fills the memory system
+// beside the issue-bound fill; each tile runs the code of its separate kernel, so every value is
+// that of separate launches.  The C2R tile takes PX rows, as many as give the fill's thread count.
+template <int L, int P, int RM, int IG, int LY, int PY, int RMY, bool HALF, int LX, int RMX, int PX>
+__global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_x_kernel(PassArgs az, PassArgs ay, PassArgs ax,
+                                                                                 FuseMap3 m) {
+    static_assert(StridedCfg<LY, PY, RMY>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
+    static_assert(C2RCfg<LX, RMX, PX>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
+    const int per = 8 * (m.A + m.B + m.C), bid = blockIdx.x, full = m.K * per;
+    int role, l;
+    if (bid < full) {
+        const int pr = bid / per, o = bid - pr * per;
+        role = o < 8 * m.A ? 0 : (o < 8 * (m.A + m.B) ? 1 : 2);
+        l = role == 0 ? pr * 8 * m.A + o
+                      : (role == 1 ? pr * 8 * m.B + (o - 8 * m.A) : pr * 8 * m.C + (o - 8 * (m.A + m.B)));
+    } else {
+        const int o = bid - full, zr = m.nzb - 8 * m.A * m.K, yr = m.nyb - 8 * m.B * m.K;
+        role = o < zr ? 0 : (o < zr + yr ? 1 : 2);
+        l = role == 0 ? 8 * m.A * m.K + o : (role == 1 ? 8 * m.B * m.K + (o - zr) : 8 * m.C * m.K + (o - zr - yr));
+    }
+    if (role == 0)
+        strided_tile<L, P, IN_FILL32H, 2, RM, IG, HALF>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
+    else if (role == 1)
+        strided_tile<LY, PY, IN_LOAD, 1, RMY, 0, HALF>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
+    else
+        c2r_tile<LX, IN_LOADH, RMX, PX>(ax, l % m.nxr, l / m.nxr);
 }
 
 // 1-point x axis (npx = 1): real part of the only coefficient of each row.

@@ -1,12 +1,15 @@
-"""The fused FP32 schedule (SS3D_OPT_FUSE, on by default): the z fill pass of batch k+1 and the y
-pass of batch k run in one launch whose blocks interleave.  The launch performs the arithmetic
-of the separate passes on the same tiles, so the fields and statistics equal those of separate
-launches bit for bit.  The cases exercise the block-role map: more fill blocks than y blocks and
-fewer (a short last batch), a tail of one role, lengths 2^5 .. 2^10, both coefficient laws, a y
-axis twice the fill's, the instance that prunes the upper half of the last radix pass (both passes
-keep at most half their axes) and the one that does not, and a grid whose z and y lengths differ
-otherwise (no fusion: separate launches either way).  The fused pass is what runs: pass kind 9
-appears in the profile only with the option on.
+"""The fused FP32 schedules (SS3D_OPT_FUSE): with 1 the z fill pass of batch k+1 and the y pass of
+batch k run in one launch whose blocks interleave (the default); with 2 the launch also holds the
+C2R x pass of batch k-1 where a three-role instance exists (fill length 2^9).  A fused launch
+performs the arithmetic of the separate passes on the same tiles, so the fields and statistics
+equal those of separate launches bit for bit.  The cases exercise the block-role map: more fill
+blocks than y blocks and fewer (a short last batch), a tail of one role, lengths 2^5 .. 2^10, both
+coefficient laws, a y axis twice the fill's, the instance that prunes the upper half of the last
+radix pass (both passes keep at most half their axes) and the one that does not, a grid whose z
+and y lengths differ otherwise (no fusion: separate launches either way), and the three-role
+launch at 512^3 and 1024x1024x512 (y twice the fill's length).  The fused pass is what runs: pass
+kind 9 appears in the profile only with the option on, kind 10 only with 2 and a fill of length
+2^9 (and plans 3 or 4, at least three batches in one call).
 """
 import sys
 
@@ -14,6 +17,7 @@ import numpy as np
 import pytest
 
 import specsim3d_amd as S
+from hipmem import DevBuf
 
 pytestmark = pytest.mark.gpu
 
@@ -49,20 +53,23 @@ def test_fused_equals_separate_launches(eng, sts, nugget, grid, nreal, igauss, f
     sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
     try:
         got = {}
-        for opt in (1, 0):
+        for opt in (2, 1, 0):
             eng.set_option(S.OPT_FUSE, opt)
             eng.profile(True)
             got[opt] = eng.realize(sp, 777, igauss, 5, nreal, nx, ny, nz, precision=S.FP32)
             eng.profile(False)
             kinds = eng.profile_read()
-            assert ("real_zy_fp32" in kinds) == (fused and opt == 1), kinds
+            assert ("real_zy_fp32" in kinds) == (fused and opt >= 1), kinds
+            assert "real_zyx_fp32" not in kinds, kinds  # no fill of length 2^9 here
     finally:
         eng.set_option(S.OPT_FUSE, 1)
         sp.close()
-    (f1, m1, v1, i1), (f0, m0, v0, i0) = got[1], got[0]
-    assert np.array_equal(f1, f0)
-    assert np.array_equal(m1, m0) and np.array_equal(v1, v0) and np.array_equal(i1, i0)
-    assert np.all(np.isfinite(f1)) and np.all(v1 > 0)
+    (f0, m0, v0, i0) = got[0]
+    for opt in (2, 1):
+        f1, m1, v1, i1 = got[opt]
+        assert np.array_equal(f1, f0)
+        assert np.array_equal(m1, m0) and np.array_equal(v1, v0) and np.array_equal(i1, i0)
+    assert np.all(np.isfinite(f0)) and np.all(v0 > 0)
 
 
 def test_fused_matches_fp64(eng):
@@ -94,18 +101,19 @@ def test_no_fusion_across_radix_plans(eng):
         eng.tuning_set(6, ly, 4)
         eng.tuning_set(7, ly, 3)
         got = {}
-        for opt in (1, 0):
+        for opt in (2, 1, 0):
             eng.set_option(S.OPT_FUSE, opt)
             eng.profile(True)
             got[opt] = eng.realize(sp, 31, igauss, 0, nreal, nx, ny, nz, precision=S.FP32)[0]
             eng.profile(False)
-            assert "real_zy_fp32" not in eng.profile_read()
+            kinds = eng.profile_read()
+            assert "real_zy_fp32" not in kinds and "real_zyx_fp32" not in kinds, kinds
     finally:
         eng.set_option(S.OPT_FUSE, 1)
         for k, v in old.items():
             eng.tuning_set(k, ly, v)
         sp.close()
-    assert np.array_equal(got[1], got[0])
+    assert np.array_equal(got[2], got[0]) and np.array_equal(got[1], got[0])
 
 
 def test_autotune_keeps_fusion(eng):
@@ -152,54 +160,134 @@ def test_fused_longer_y_pass(eng, sts, grid, nreal, rmz):
         eng.tuning_set(6, lz, rmz)
         eng.tuning_set(7, ly, 3)
         got = {}
-        for opt in (1, 0):
+        for opt in (2, 1, 0):
             eng.set_option(S.OPT_FUSE, opt)
             eng.profile(True)
             got[opt] = eng.realize(sp, 99, 1, 2, nreal, nx, ny, nz, precision=S.FP32)
             eng.profile(False)
             kinds = eng.profile_read()
-            assert ("real_zy_fp32" in kinds) == (opt == 1), kinds
+            assert ("real_zy_fp32" in kinds) == (opt >= 1), kinds
+            assert "real_zyx_fp32" not in kinds, kinds  # x lengths without a three-role instance
     finally:
         eng.set_option(S.OPT_FUSE, 1)
         eng.tuning_set(6, lz, old[6])
         eng.tuning_set(7, ly, old[7])
         sp.close()
-    (f1, m1, v1, _), (f0, m0, v0, _) = got[1], got[0]
-    assert np.array_equal(f1, f0)
-    assert np.array_equal(m1, m0) and np.array_equal(v1, v0)
+    (f0, m0, v0, _) = got[0]
+    for opt in (2, 1):
+        f1, m1, v1, _ = got[opt]
+        assert np.array_equal(f1, f0)
+        assert np.array_equal(m1, m0) and np.array_equal(v1, v0)
 
 
-def test_fused_full_size():
+def full_size_runs(e, sts, nugget, grid, nreal, opts, seed, plans):
+    """Realizations 0..nreal-1 in one device call (so that the schedule sees every batch) under each
+    SS3D_OPT_FUSE value in `opts`, with the radix plans `plans` ({(kind, logn): rm}) in place:
+    {opt: (field, mean, var, maximag)}, {opt: pass kinds run}, and the spectrum."""
+    nx, ny, nz = grid
+    ncell = nx * ny * nz
+    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
+    sp = e.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
+    old = {k: e.tuning_get(*k) for k in plans}
+    got, kinds = {}, {}
+    try:
+        for k, rm in plans.items():
+            e.tuning_set(*k, rm)
+        with DevBuf(nreal * ncell * 4) as out, DevBuf(nreal * 3 * 8) as stats:
+            for opt in opts:
+                say(f"[fused] {grid} in {npg}, {nreal} realizations, plans {plans}, option {opt}")
+                e.set_option(S.OPT_FUSE, opt)
+                e.profile(True)
+                e.realize_device(sp, seed, 1, 0, nreal, nx, ny, nz, S.FP32, out.ptr, stats.ptr)
+                e.profile(False)
+                kinds[opt] = e.profile_read()
+                st = stats.get(np.float64, nreal * 3).reshape(nreal, 3)
+                f = out.get(np.float32, nreal * ncell).reshape(nreal, nz, ny, nx)
+                got[opt] = (f, st[:, 0].copy(), st[:, 1].copy(), st[:, 2].copy())
+    except BaseException:
+        sp.close()
+        raise
+    finally:
+        e.set_option(S.OPT_FUSE, 1)
+        for k, rm in old.items():
+            e.tuning_set(*k, rm)
+    return sp, got, kinds
+
+
+def assert_same_as_separate(got):
+    f0, m0, v0, i0 = got[0]
+    for opt in sorted(got):
+        f1, m1, v1, i1 = got[opt]
+        assert np.array_equal(f1, f0), opt
+        assert np.array_equal(m1, m0) and np.array_equal(v1, v0) and np.array_equal(i1, i0), opt
+
+
+# (fill plan, x plan) at 512^3: the y pass takes the fill's plan
+PLANS_512 = [(4, 3), (3, 4)]
+
+
+@pytest.mark.parametrize("rmz,rmx", PLANS_512)
+def test_fused_full_size(rmz, rmx):
     """The 256^3 output grid in a 512^3 periodic grid (exponential 0.7 + nugget 0.3, ranges 80/40/10,
-    az 30, dip 15, igauss=1), 17 realizations: the fused launches (two batches of 8 per device
-    round trip, then one realization alone) give the fields of separate launches bit for bit, and
-    realizations that went through the fused launch match the FP64 path to 1e-5 relative RMS."""
+    az 30, dip 15, igauss=1), 33 realizations in batches of 8, 8, 8, 8, 1: three-role launches
+    (fill of batch k, y pass of batch k-1, C2R x pass of batch k-2; the last with a fill of one
+    realization) and two-role launches give the fields and statistics of separate launches bit for
+    bit, with C2R tiles of 16 rows (plan 3) and 32 rows (plan 4); realizations that went through
+    the three-role launch (3, 12) and the last one match the FP64 path to 1e-5 relative RMS."""
+    sts = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]
+    plans = {(6, 9): rmz, (7, 9): rmz, (8, 8): rmx}
+    with S.Engine(0) as e:
+        sp, got, kinds = full_size_runs(e, sts, 0.3, (256, 256, 256), 33, (2, 1, 0), 69069, plans)
+        try:
+            assert kinds[2].get("real_zyx_fp32", (0, 0, 0))[2] == 3, kinds[2]
+            assert "real_zy_fp32" in kinds[1] and "real_zyx_fp32" not in kinds[1], kinds[1]
+            assert "real_zy_fp32" not in kinds[0] and "real_zyx_fp32" not in kinds[0], kinds[0]
+            say("[fused] full size: fp64 of realizations 3, 12 and 32")
+            f64 = {r: e.realize(sp, 69069, 1, r, 1, 256, 256, 256, precision=S.FP64)[0][0] for r in (3, 12, 32)}
+        finally:
+            sp.close()
+    assert_same_as_separate(got)
+    for r, ref in f64.items():
+        rel = np.sqrt(np.mean((got[2][0][r].astype(np.float64) - ref) ** 2) / np.mean(ref ** 2))
+        say(f"realization {r}: relative RMS {rel:.2e}")
+        assert rel < 1e-5
+    v = got[2][2]
+    assert 0.5 < v.min() and v.max() < 1.5
+
+
+def test_fused_three_roles_longer_axes():
+    """512x512x256 in 1024x1024x512 (Gaussian sill 1, ranges 150/60/20, az 120, dip -20, igauss=1):
+    the three-role launch with a y pass twice the fill's length (2^10, plan 3, beside 2^9, plan 4)
+    and a C2R pass of complex length 2^9 (plan 4, tiles of 8 rows); 20 realizations (batches 8, 8,
+    4) give the fields and statistics of separate launches bit for bit."""
+    sts = [(3, 1.0, 150.0, 60.0, 20.0, 120.0, -20.0, 0.0)]
+    plans = {(6, 9): 4, (7, 10): 3, (8, 9): 4}
+    with S.Engine(0) as e:
+        sp, got, kinds = full_size_runs(e, sts, 0.0, (512, 512, 256), 20, (2, 0), 4711, plans)
+        sp.close()
+    assert kinds[2].get("real_zyx_fp32", (0, 0, 0))[2] == 1, kinds[2]
+    assert "real_zy_fp32" not in kinds[0] and "real_zyx_fp32" not in kinds[0], kinds[0]
+    assert_same_as_separate(got)
+    v = got[2][2]
+    assert 0.3 < v.min() and v.max() < 2.0
+
+
+def test_autotune_picks_three_roles():
+    """With SS3D_OPT_FUSE 2, the autotuner's plans at 512^3 have a three-role launch, and 24
+    realizations after it run through one."""
     sts = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]
     npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, 256, 256, 256, 1)
-    assert npg == (512, 512, 512)
-    say("[fused] full size 256^3 in 512^3, 17 realizations")
     with S.Engine(0) as e:
         sp = e.spectrum(sts, 0.3, 1.0, 1.0, 1.0, *npg)
         try:
-            got = {}
-            for opt in (1, 0):
-                e.set_option(S.OPT_FUSE, opt)
+            e.set_option(S.OPT_FUSE, 2)
+            plan = e.autotune(sp, 256, 256, 256, 1, reps=1)
+            say(f"[fused] autotune 512^3: {plan}")
+            with DevBuf(24 * 256 ** 3 * 4) as out:
                 e.profile(True)
-                got[opt] = e.realize(sp, 69069, 1, 0, 17, 256, 256, 256, precision=S.FP32)
+                e.realize_device(sp, 7, 1, 0, 24, 256, 256, 256, S.FP32, out.ptr)
                 e.profile(False)
                 kinds = e.profile_read()
-                assert ("real_zy_fp32" in kinds) == (opt == 1), kinds
-            e.set_option(S.OPT_FUSE, 1)
-            say("[fused] full size: fp64 of realizations 3 and 12")
-            f64 = {r: e.realize(sp, 69069, 1, r, 1, 256, 256, 256, precision=S.FP64)[0][0] for r in (3, 12)}
         finally:
-            e.set_option(S.OPT_FUSE, 1)
             sp.close()
-    (f1, m1, v1, i1), (f0, m0, v0, i0) = got[1], got[0]
-    assert np.array_equal(f1, f0)
-    assert np.array_equal(m1, m0) and np.array_equal(v1, v0)
-    for r, ref in f64.items():
-        rel = np.sqrt(np.mean((f1[r].astype(np.float64) - ref) ** 2) / np.mean(ref ** 2))
-        say(f"realization {r}: relative RMS {rel:.2e}")
-        assert rel < 1e-5
-    assert 0.5 < v1.min() and v1.max() < 1.5
+    assert "real_zyx_fp32" in kinds, (plan, kinds)
