@@ -625,6 +625,85 @@ struct C2RCfg {
     static constexpr size_t lds_elems() { return (size_t)ROW * P; }
 };
 
+// X[N-k] for k = g + r G from the lanes of the same row (G <= 64: a row lies inside one wave):
+// register EPT-1-r of lane G-g of the row (g >= 1), or this lane's register EPT-r (g = 0, r >= 1)
+// -- each value read once from memory, exchanged
+template <int EPT, int G>
+__device__ __forceinline__ void c2r_mirror(const f2* xa, f2* xb, const int g) {
+    static_assert(G <= 64, "a row inside one wave");
+    const int lane = threadIdx.x & 63;
+    const int plane = lane - g + ((G - g) & (G - 1));
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        const f2 t = xa[EPT - 1 - r];
+        const f2 sh = mk(__shfl(t.x, plane, 64), __shfl(t.y, plane, 64));
+        xb[r] = g != 0 ? sh : (r == 0 ? mk(xa[0].y, 0.f) : xa[(EPT - r) & (EPT - 1)]);
+    }
+}
+
+// Pass-0 operands of the length-N inverse transform of a real row of 2N: v[r] = E + i O from
+// X[k] (xa) and X[N-k] (xb), k = g + r G; tw2 reads the W_{2N} table.
+template <int EPT, int G, typename TW>
+__device__ __forceinline__ void c2r_operands(const f2* xa, const f2* xb, TW tw2, const int g, f2* v) {
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+        const int k = g + r * G;
+        const f2 A = xa[r];
+        const f2 B = mk(xb[r].x, -xb[r].y);  // conj X[N-k]
+        const f2 E = add(A, B);
+        const f2 O = mul(tw2(k), sub(A, B));
+        v[r] = mk(E.x - O.y, E.y + O.x);
+    }
+}
+
+// The radix passes of the rows' transforms, exchanging through LDS between passes (the block
+// waits at a barrier before each exchange after the first, and after each write).
+template <int L, int RM, int PR, typename TW>
+__device__ __forceinline__ void c2r_passes(f2* v, f2* lds, const int q, const int g, TW tw) {
+    using C = C2RCfg<L, RM, PR>;
+    constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M;
+    [&]<int... J>(std::integer_sequence<int, J...>) {
+        (
+            [&] {
+                constexpr int Jc = J;
+                if constexpr (Jc > 0) {
+                    constexpr int LRp = rlog(L, Jc - 1, RM), NBp = EPT >> LRp;
+                    constexpr int LR = rlog(L, Jc, RM), NB = EPT >> LR;
+                    if constexpr (Jc > 1) __syncthreads();
+#pragma unroll
+                    for (int i = 0; i < NBp; ++i)
+#pragma unroll
+                        for (int r = 0; r < (1 << LRp); ++r)
+                            lds[q * C::ROW + C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r))] = v[i * (1 << LRp) + r];
+                    __syncthreads();
+#pragma unroll
+                    for (int i = 0; i < NB; ++i)
+#pragma unroll
+                        for (int r = 0; r < (1 << LR); ++r)
+                            v[i * (1 << LR) + r] = lds[q * C::ROW + C::ppos(g + G * i + r * (N >> LR))];
+                }
+                radix_pass<L, Jc, G, RM>(v, g, tw);
+            }(),
+            ...);
+    }(std::make_integer_sequence<int, M>{});
+}
+
+// Block statistics (sum, sum of squares; max|Im| = 0 for a real-output transform): wave sums
+// into red[0 .. 2 NW) (the exchange buffer, once every wave is past its last LDS read: no static
+// LDS, so the fused launch keeps two blocks per CU); on return every thread may read them.
+template <int NW>
+__device__ __forceinline__ void c2r_wave_sums(double s1, double s2, double* red) {
+    s1 = wsum(s1);
+    s2 = wsum(s2);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        red[wid] = s1;
+        red[NW + wid] = s2;
+    }
+    __syncthreads();
+}
+
 // One block of the C2R pass: rows rblock*P .. +P-1 of batch member b.
 template <int L, int IN, int RM, int PR = 0>
 __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, const int b) {
@@ -651,16 +730,7 @@ __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, co
                 xa[r] = mk(t.x, t.y);
             }
             if constexpr (G <= 64) {
-                // a row lies inside one wave: X[N-k] for k = g + r G is register EPT-1-r of lane G-g of the
-                // row (g >= 1), or this lane's register EPT-r (g = 0, r >= 1) -- read once, exchanged
-                const int lane = threadIdx.x & 63;
-                const int plane = lane - g + ((G - g) & (G - 1));
-#pragma unroll
-                for (int r = 0; r < EPT; ++r) {
-                    const f2 t = xa[EPT - 1 - r];
-                    const f2 sh = mk(__shfl(t.x, plane, 64), __shfl(t.y, plane, 64));
-                    xb[r] = g != 0 ? sh : (r == 0 ? mk(xa[0].y, 0.f) : xa[(EPT - r) & (EPT - 1)]);
-                }
+                c2r_mirror<EPT, G>(xa, xb, g);
             } else {
 #pragma unroll
                 for (int r = 0; r < EPT; ++r) {
@@ -683,44 +753,13 @@ __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, co
                 }
             }
         }
-#pragma unroll
-        for (int r = 0; r < EPT; ++r) {
-            const int k = g + r * G;
-            const f2 A = xa[r];
-            const f2 B = mk(xb[r].x, -xb[r].y);  // conj X[N-k]
-            const f2 E = add(A, B);
-            const f2 O = mul(__ldg(tw2 + k), sub(A, B));
-            v[r] = mk(E.x - O.y, E.y + O.x);
-        }
+        c2r_operands<EPT, G>(xa, xb, TwGlobal{tw2}, g, v);
     } else {
 #pragma unroll
         for (int r = 0; r < EPT; ++r) v[r] = mk(0.f, 0.f);
     }
 
-    [&]<int... J>(std::integer_sequence<int, J...>) {
-        (
-            [&] {
-                constexpr int Jc = J;
-                if constexpr (Jc > 0) {
-                    constexpr int LRp = rlog(L, Jc - 1, RM), NBp = EPT >> LRp;
-                    constexpr int LR = rlog(L, Jc, RM), NB = EPT >> LR;
-                    if constexpr (Jc > 1) __syncthreads();
-#pragma unroll
-                    for (int i = 0; i < NBp; ++i)
-#pragma unroll
-                        for (int r = 0; r < (1 << LRp); ++r)
-                            lds[q * C::ROW + C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r))] = v[i * (1 << LRp) + r];
-                    __syncthreads();
-#pragma unroll
-                    for (int i = 0; i < NB; ++i)
-#pragma unroll
-                        for (int r = 0; r < (1 << LR); ++r)
-                            v[i * (1 << LR) + r] = lds[q * C::ROW + C::ppos(g + G * i + r * (N >> LR))];
-                }
-                radix_pass<L, Jc, G, RM>(v, g, TwGlobal{tw});
-            }(),
-            ...);
-    }(std::make_integer_sequence<int, M>{});
+    c2r_passes<L, RM, PR>(v, lds, q, g, TwGlobal{tw});
 
     double s1 = 0.0, s2 = 0.0;
     if (active) {
@@ -755,25 +794,15 @@ __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, co
                 }
             }
     }
-    // block statistics (sum, sum of squares, max|Im| = 0 for a real-output transform)
-    // (in the exchange buffer, once every wave is past its last LDS read: no static LDS, so the
-    // fused launch keeps two blocks per CU).  A block of PR rows writes the partials of the
-    // standard blocks it spans (their waves, in their order), so that the statistics do not depend
-    // on the launch that ran the pass; a.nblocks counts standard blocks.
+    // A block of PR rows writes the partials of the standard blocks it spans (their waves, in
+    // their order), so that the statistics do not depend on the launch that ran the pass;
+    // a.nblocks counts standard blocks.
     constexpr int NW = (C::NT + 63) / 64;
     constexpr int SW = (C2RCfg<L, RM>::NT + 63) / 64, NSUB = NW / SW;
     static_assert(NW % SW == 0 && C::P % C2RCfg<L, RM>::P == 0, "whole standard blocks");
     static_assert(C::lds_elems() * sizeof(f2) >= 2 * NW * sizeof(double), "room for the block sums");
-    s1 = wsum(s1);
-    s2 = wsum(s2);
-    const int lane = tid & 63, wid = tid >> 6;
     double* red = reinterpret_cast<double*>(fast_smem);
-    __syncthreads();
-    if (lane == 0) {
-        red[wid] = s1;
-        red[NW + wid] = s2;
-    }
-    __syncthreads();
+    c2r_wave_sums<NW>(s1, s2, red);
     if (tid == 0) {
         for (int sb = 0; sb < NSUB; ++sb) {
             const int blk = rblock * NSUB + sb;
@@ -796,13 +825,136 @@ __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_kernel(PassArgs a) {
     c2r_tile<L, IN, RM>(a, blockIdx.x, blockIdx.y);
 }
 
+// 8-byte raw buffer accesses (non-temporal), out-of-range offsets as for the 16-byte ones
+__device__ __forceinline__ f2 bload8(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const unsigned int __attribute__((ext_vector_type(2))) w = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 2);
+    return mk(__uint_as_float(w.x), __uint_as_float(w.y));
+}
+__device__ __forceinline__ void bstore8(f2 v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const unsigned int __attribute__((ext_vector_type(2))) w = {__float_as_uint(v.x), __float_as_uint(v.y)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, rs, off, 0, 2);
+}
+__device__ __forceinline__ void bstore8d(double v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const uint64_t u = __double_as_longlong(v);
+    const unsigned int __attribute__((ext_vector_type(2))) w = {(unsigned int)u, (unsigned int)(u >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(w, rs, off, 0, 2);
+}
+
+// The C2R pass from the y output (IN_LOADH) over members b0 .. b1-1 of one tile of P rows (rows in
+// one wave, an even cropped length, 8-byte aligned output rows): the rows of member b+1 are loaded
+// while member b is transformed and stored.  Every access goes through a buffer resource with
+// out-of-range offsets for rows past the end, members past the last and the lanes that write no
+// partial, so every wave issues the same memory instructions per member and the wait for the
+// prefetched rows never includes the stores of the member before.  Both twiddle tables are copied
+// to LDS behind the exchange buffer (a load from memory would complete only after the prefetch).
+// Used for such passes whatever the members per block (one included) and inside the three-role
+// launch, so that a realization's values never depend on the schedule.  A tile of PR rows writes
+// the partials of the standard tiles it spans, as c2r_tile does.
+template <int L, int RM, int PR = 0>
+struct C2RRowsCfg {
+    using C = C2RCfg<L, RM, PR>;
+    static constexpr size_t lds_bytes() { return (C::lds_elems() + 2 * (size_t)C::N) * sizeof(f2); }
+};
+template <int L, int RM, int PR>
+__device__ __forceinline__ void c2r_rows(const PassArgs& a, const int rblock, const int b0, const int b1) {
+    using C = C2RCfg<L, RM, PR>;
+    constexpr int N = C::N, EPT = C::EPT, G = C::G, P = C::P, M = C::M, NW = (C::NT + 63) / 64;
+    constexpr int SW = (C2RCfg<L, RM>::NT + 63) / 64, NSUB = NW / SW;
+    static_assert(G <= 64 && M > 0, "rows inside one wave, at least one radix pass");
+    static_assert(NW % SW == 0 && C::P % C2RCfg<L, RM>::P == 0, "whole standard tiles");
+    static_assert(C::lds_elems() * sizeof(f2) >= 2 * NW * sizeof(double), "room for the block sums");
+    f2* lds = reinterpret_cast<f2*>(fast_smem);
+    f2* twl = lds + C::lds_elems();  // W_N (N entries), then the first N entries of W_{2N}
+    const int tid = threadIdx.x;
+    const int g = tid % G, q = tid / G;
+    const bool active = rblock * P + q < a.nrows;
+    {
+        const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);
+        for (int m = tid; m < 2 * N; m += C::NT) twl[m] = __ldg(tw + m);
+    }
+    const uint32_t ibase = (uint32_t)q * (uint32_t)a.in_row + g;
+    f2 nxt[EPT];
+    auto load = [&](int b) {  // b >= b1: zeros, no memory access
+        const int bb = min(b, b1 - 1);
+        const __amdgpu_buffer_rsrc_t rs =
+            rsrc(reinterpret_cast<const f2*>(a.in) + (int64_t)bb * a.in_bstride + (int64_t)rblock * P * a.in_row);
+        const bool ok = active && b < b1;
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) nxt[r] = bload8(rs, ok ? (ibase + r * G) * 8u : kOob);
+    };
+    constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
+    __syncthreads();  // twiddles in LDS (and, in a fused launch, every wave past its role's setup)
+    load(b0);
+    {
+        // as many out-of-range (dropped) stores as a member issues: the loop is entered with the
+        // same vector-memory counter history as from its back edge
+        const __amdgpu_buffer_rsrc_t rs = rsrc(a.out);
+#pragma unroll
+        for (int e = 0; e < EPT + 3 * NSUB; ++e) bstore8(mk(0.f, 0.f), rs, kOob + 8u * e);
+    }
+    const uint32_t obase = (uint32_t)q * (uint32_t)a.out_row;
+    const int nx = a.n_out;
+    for (int b = b0; b < b1; ++b) {
+        f2 xa[EPT], xb[EPT], v[EPT];
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) xa[r] = nxt[r];
+        load(b + 1);  // in flight while member b is transformed and stored
+        if (b > b0) __syncthreads();  // the block sums of member b-1 are read
+        c2r_mirror<EPT, G>(xa, xb, g);
+        // column 0 = g0 + i gN (packed Nyquist column): X[0] = Re, X[N] = Im
+        if (g == 0) xa[0].y = 0.f;
+        c2r_operands<EPT, G>(xa, xb, TwLds{twl + N}, g, v);
+        c2r_passes<L, RM, PR>(v, lds, q, g, TwLds{twl});
+        double s1 = 0.0, s2 = 0.0;
+        const __amdgpu_buffer_rsrc_t ro =
+            rsrc(reinterpret_cast<float*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)rblock * P * a.out_row);
+#pragma unroll
+        for (int i = 0; i < NBl; ++i)
+#pragma unroll
+            for (int r = 0; r < (1 << LRl); ++r) {
+                const int m = out_pos<L, M - 1, RM>(g + G * i, r);
+                const f2 z = v[i * (1 << LRl) + r];
+                const bool ok = active && 2 * m + 1 < nx;
+                bstore8(z, ro, ok ? (obase + 2 * m) * 4u : kOob);
+                if (ok) {
+                    s1 += (double)z.x + (double)z.y;
+                    s2 += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+                }
+            }
+        double* red = reinterpret_cast<double*>(fast_smem);
+        c2r_wave_sums<NW>(s1, s2, red);
+#pragma unroll
+        for (int sb = 0; sb < NSUB; ++sb) {
+            const int blk = rblock * NSUB + sb;
+            double t1 = 0, t2 = 0;
+            for (int w = sb * SW; w < (sb + 1) * SW; ++w) {
+                t1 += red[w];
+                t2 += red[NW + w];
+            }
+            const __amdgpu_buffer_rsrc_t rp = rsrc(a.partial + 3 * ((int64_t)b * a.nblocks + min(blk, a.nblocks - 1)));
+            const uint32_t po = tid == 0 && blk < a.nblocks ? 0u : kOob;
+            bstore8d(t1, rp, po);
+            bstore8d(t2, rp, po + 8u);
+            bstore8d(0.0, rp, po + 16u);
+        }
+    }
+}
+
+// The C2R pass from the y output over a.bpb members per block (grid y: member groups).
+template <int L, int RM>
+__global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_loop_kernel(PassArgs a) {
+    const int b0 = blockIdx.y * a.bpb;
+    c2r_rows<L, RM, 0>(a, blockIdx.x, b0, min(b0 + a.bpb, a.nbatch));
+}
+
 // ------------------------------------------------------------ fill, y and C2R passes in one launch
 // The fill of batch k, the y pass of batch k-1 and the C2R x pass of batch k-2 (the x role reads
 // the y output of two launches before, which is double-buffered).  Roles interleave in periods
 // of 8A fill, 8B y and 8C x blocks (A:B:C the ratio of their block counts), K times, then the
-// rest of each.  The x role is memory-bound and light on VALU, so it fills the memory system
-// beside the issue-bound fill; each tile runs the code of its separate kernel, so every value is
-// that of separate launches.  The C2R tile takes PX rows, as many as give the fill's thread count.
+// rest of each.  Each tile runs the code of its separate kernel (c2r_rows for the x role, over
+// one member), so every value is that of separate launches.  The C2R tile takes PX rows, as many
+// as give the fill's thread count.  (Measured no faster than the two-role launch followed by the
+// separate x pass: the launch is bound by instruction issue, and the x role adds its own.)
 template <int L, int P, int RM, int IG, int LY, int PY, int RMY, bool HALF, int LX, int RMX, int PX>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_x_kernel(PassArgs az, PassArgs ay, PassArgs ax,
                                                                                  FuseMap3 m) {
@@ -825,7 +977,7 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_x_kernel
     else if (role == 1)
         strided_tile<LY, PY, IN_LOAD, 1, RMY, 0, HALF>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
     else
-        c2r_tile<LX, IN_LOADH, RMX, PX>(ax, l % m.nxr, l / m.nxr);
+        c2r_rows<LX, RMX, PX>(ax, l % m.nxr, l / m.nxr, l / m.nxr + 1);
 }
 
 // 1-point x axis (npx = 1): real part of the only coefficient of each row.

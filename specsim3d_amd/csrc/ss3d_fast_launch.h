@@ -115,6 +115,13 @@ hipError_t fused_rm(int L, int LY, int nblocks, const PassArgs& az, const PassAr
                      : fused_dispatch<RM, 0>(L, LY, nblocks, az, ay, m, s, seq{});
 }
 
+// the member-loop C2R code (c2r_rows) serves a pass from the y output whose buffer offsets fit
+// it: an even cropped length, even output rows, 8-byte aligned output, tiles of P rows within 2 GiB
+inline bool c2r_rows_fit(const PassArgs& a, int P) {
+    return ((a.n_out | a.out_row | a.out_bstride) & 1) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 7) == 0 &&
+           (int64_t)P * a.in_row * 8 < 0x7fffffffLL && (int64_t)P * a.out_row * 4 < 0x7fffffffLL;
+}
+
 // ---- fill, y and C2R x passes in one launch (fill_load_x_kernel).  Instantiated for fills of
 // length 2^kFuse3Log with plan 3 or 4 whose z and y passes keep at most half their axes (HALF), a
 // y pass of the fill's length and plan or of twice its length with plan 3, and a C2R pass of
@@ -141,10 +148,10 @@ constexpr bool fused3_ok() {
             return false;
         } else {
             using CX = fast::C2RCfg<LX, RMX, PX>;
-            constexpr size_t lx = CX::lds_elems() * sizeof(float2);
+            constexpr size_t lx = fast::C2RRowsCfg<LX, RMX, PX>::lds_bytes();
             constexpr size_t lds = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
             constexpr bool ypair = LY == L ? RMY == RM : (RMY == 3 && fused_up<L, RM>());
-            return ypair && CY::NT == C::NT && CX::NT == C::NT && 2 * (lds > lx ? lds : lx) <= 163840;
+            return ypair && CX::G <= 64 && CY::NT == C::NT && CX::NT == C::NT && 2 * (lds > lx ? lds : lx) <= 163840;
         }
     }
 }
@@ -158,11 +165,10 @@ hipError_t fused3_L(int nblocks, const PassArgs& az, const PassArgs& ay, const P
         using C = fast::StridedCfg<L, strided_P(L), RM>;
         using CY = fast::StridedCfg<LY, strided_P(LY), RMY>;
         constexpr int PX = fused3_rows<LX, RMX, C::NT>();
-        using CX = fast::C2RCfg<LX, RMX, PX>;
         static_assert(3 * sizeof(PassArgs) + sizeof(FuseMap3) <= 4096, "kernel arguments");
         static bool attr = false;
         const auto k = fast::fill_load_x_kernel<L, strided_P(L), RM, IG, LY, strided_P(LY), RMY, true, LX, RMX, PX>;
-        constexpr size_t lx = CX::lds_elems() * sizeof(float2);
+        constexpr size_t lx = fast::C2RRowsCfg<LX, RMX, PX>::lds_bytes();
         constexpr size_t lzy = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
         constexpr size_t lds = lzy > lx ? lzy : lx;
         if (lds > 65536 && !attr) {
@@ -233,6 +239,13 @@ hipError_t fused3_rm(int L, int LY, int LX, int rmx, int nblocks, const PassArgs
 template <int L, int IN, int RM>
 hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using C = fast::C2RCfg<L, RM>;
+    if constexpr (IN == IN_LOADH && C::G <= 64 && C::M > 0) {
+        if (c2r_rows_fit(a, C::P)) {
+            static bool attr_loop = false;
+            return launch_k(fast::c2r_loop_kernel<L, RM>, fast::C2RRowsCfg<L, RM>::lds_bytes(), C::NT, nblocks,
+                            (nbatch + a.bpb - 1) / a.bpb, a, s, attr_loop);
+        }
+    }
     static bool attr = false;
     return launch_k(fast::c2r_kernel<L, IN, RM>, C::lds_elems() * sizeof(float2), C::NT, nblocks, nbatch, a, s, attr);
 }

@@ -566,10 +566,11 @@ int prep_pass(Pass& p, int nbatch, hipStream_t strm, DevBuf& partial, int& group
     }
     p.a.nblocks = p.nblocks;
     if (p.nblocks <= 0) return fail(SS3D_EINVAL, "empty pass");
-    // FP32 strided passes: a block serves bpb batch members, as many as keep >= kFillBlocks blocks
+    // FP32 strided passes and the C2R pass from the y output: a block serves bpb batch members, as
+    // many as keep >= kFillBlocks blocks
     p.a.nbatch = nbatch;
     p.a.bpb = 1;
-    if (p.fast == 1)
+    if (p.fast == 1 || (p.fast == 2 && p.in == IN_LOADH))
         p.a.bpb = std::max(1, std::min(nbatch, (int)((int64_t)p.nblocks * nbatch / kFillBlocks)));
     groups = (nbatch + p.a.bpb - 1) / p.a.bpb;
     if (p.fast == 1 && p.in == IN_LOAD) {  // buffer addressing: byte offsets inside a tile below 2^31
@@ -1278,8 +1279,11 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     const int rmy_fused = lz == ly ? pz.rm : 3;
     const bool fuse2 = staged && has_z && has_y && !pipelined && c->fuse >= 1 && nbat >= 2 && pz.var == 0 &&
                        py.var == 0 && fast_same_plan(ly, py.rm, rmy_fused) && fast_fusable(lz, pz.rm, ly, rmy_fused);
-    // the C2R x pass as a third role: its plan has a three-role instance beside these two
-    const int rows3 = fuse2 && allow_fuse3 && c->fuse >= 2 && nbat >= 3 && px.fast == 2 && px.in == IN_LOADH
+    // the C2R x pass as a third role: its plan has a three-role instance beside these two, and its
+    // output layout is one the separate pass runs through the same member-loop code (even rows
+    // and members, 8-byte aligned)
+    const bool x_rows = ((nx | ncrop) & 1) == 0 && (reinterpret_cast<uintptr_t>(dout) & 7) == 0;
+    const int rows3 = fuse2 && allow_fuse3 && c->fuse >= 2 && nbat >= 3 && px.fast == 2 && px.in == IN_LOADH && x_rows
                           ? fast_fused3_rows(lz, pz.rm, ly, rmy_fused, px.a.logn, px.rm)
                           : 0;
     if (rows3 > 0) {

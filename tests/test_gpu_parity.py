@@ -103,6 +103,38 @@ def test_realization_offsets_are_independent(eng, precision, igauss):
     assert np.array_equal(full[1:4], mid)
 
 
+# (output grid, realizations, C2R radix plan): 256^3 in 512^3 (8 members per block); 250x130x110
+# in 512x512x256: 14300 rows in tiles of 8 (plan 3: 6 members per block, then 2) or 16 rows (plan
+# 4: 3, 3 and 2 members), the last tile ragged
+C2R_LOOP = [((256, 256, 256), 8, 4), ((250, 130, 110), 8, 3), ((250, 130, 110), 8, 4)]
+
+
+@pytest.mark.parametrize("grid,nreal,rm", C2R_LOOP)
+def test_c2r_member_loop_equals_single_members(eng, grid, nreal, rm):
+    """The C2R pass over several batch members per block (the rows of member b+1 are loaded while
+    member b is transformed) gives every realization and its statistics bit for bit as the pass
+    over one member computes them (a realization alone): one launch over all members here."""
+    sts = [(2, 0.7, 80, 40, 10, 30, 15, 0)]
+    npg = S.periodic_grid(sts, 1, 1, 1, *grid, 1)
+    lx = npg[0].bit_length() - 1
+    say(f"[c2r loop] grid {grid} periodic {npg} nreal {nreal} plan {rm}")
+    old = eng.tuning_get(8, lx - 1)
+    sp = eng.spectrum(sts, 0.3, 1, 1, 1, *npg)
+    try:
+        eng.tuning_set(8, lx - 1, rm)
+        eng.profile(True)
+        f, m, v, _ = eng.realize(sp, 69069, 1, 0, nreal, *grid, S.FP32)
+        eng.profile(False)
+        assert eng.profile_read()["real_x_c2r_fp32"][2] == 1  # one launch over every member
+        for r in range(nreal):
+            one, m1, v1, _ = eng.realize(sp, 69069, 1, r, 1, *grid, S.FP32)
+            assert np.array_equal(f[r], one[0]), r
+            assert m[r] == m1[0] and v[r] == v1[0], r
+    finally:
+        eng.tuning_set(8, lx - 1, old)
+        sp.close()
+
+
 def test_256cubed_full_size_fp32_vs_fp64(eng):
     """256^3 in a 512^3 periodic grid (exponential 0.7 + nugget 0.3, ranges 80/40/10, az 30, dip 15,
     igauss=1): FP32 vs FP64 from the same draws, sill reproduced by the sample variance."""
