@@ -147,13 +147,11 @@ int ss3d_realize_device_noalloc(ss3d_context* ctx, const ss3d_spectrum* spec, in
 
 /* Per-pass timing with HIP events recorded on the context stream around every pass launch.
  * Pass kinds: 0..2 spectrum x/y/z (FP64), 3..5 FP64 realization x/y/z, 6..8 FP32 realization
- * z/y/x, 9 FP32 z fill beside the y pass of the previous batch (SS3D_OPT_FUSE 1), 10 the same
- * launch also holding the C2R x pass of the batch before (SS3D_OPT_FUSE 2).  enable=1 clears and
- * starts recording, enable=0 stops.  profile_read synchronises the
+ * z/y/x, 9 FP32 z fill beside the y pass of the previous batch (SS3D_OPT_FUSE).  enable=1 clears and starts recording, enable=0 stops.  profile_read synchronises the
  * stream and returns, per kind, total milliseconds, algorithmic HBM bytes (bytes each pass must
  * read and write) and launch count.  No counterpart in the reference (its timers are the wall
  * clocks printed at f90:1299-1300, 1350-1351). */
-#define SS3D_NPASS_KINDS 11
+#define SS3D_NPASS_KINDS 10
 int ss3d_context_profile(ss3d_context* ctx, int enable);
 int ss3d_context_profile_read(ss3d_context* ctx, double* ms, double* bytes, int64_t* launches, int nkinds);
 
@@ -187,11 +185,7 @@ int ss3d_tuning_set_tile(ss3d_context* ctx, int kind, int logn, int rm, int vari
 /* SS3D_OPT_FUSE (default 1): FP32 realizations whose z and y axes have one length (or a y axis
  * twice as long) run the fill pass of batch k+1 and the y pass of batch k in one launch whose
  * blocks interleave (arithmetic-bound and memory-bound tiles share each CU), where the passes' radix
- * plans have such a launch; 2 also puts the C2R x pass of batch k-1 in that launch where a
- * three-role instance exists (fill length 2^9 with plans 3 or 4, at least three batches per call;
- * no faster than 1 on the 512^3 and 1024x1024x512 grids: the launch is issue-bound and the x pass
- * adds its instructions), and makes the autotuner prefer x plans that have one; 0 runs the passes
- * as separate launches.  Same values and statistics. */
+ * plans have such a launch; 0 runs them as separate launches.  Same values. */
 #define SS3D_OPT_FUSE 3
 /* SS3D_OPT_BATCH_AUTOTUNE (default 1): ss3d_batch_create autotunes every distinct grid of its
  * jobs; 0 keeps the context's tuning table as it is (for comparisons with untuned plans). */

@@ -176,9 +176,9 @@ SIGNATURES = {
     "ss3d_format_es": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
 }
 
-NPASS_KINDS = 11
+NPASS_KINDS = 10
 PASS_KINDS = ["spectrum_x_fp64", "spectrum_y_fp64", "spectrum_z_fp64", "real_x_fp64", "real_y_fp64", "real_z_fp64",
-              "real_z_fp32", "real_y_fp32", "real_x_c2r_fp32", "real_zy_fp32", "real_zyx_fp32"]
+              "real_z_fp32", "real_y_fp32", "real_x_c2r_fp32", "real_zy_fp32"]
 
 
 def lib():

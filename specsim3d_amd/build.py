@@ -34,6 +34,7 @@ UNITS = {
     "ss3d_k32_r3.hip": [],
     "ss3d_k32_r4.hip": [],
     "ss3d_k32_r5.hip": [],
+    "ss3d_k32_c2r.hip": [],
     "ss3d_host.hip": ["-ffp-contract=off"],
 }
 

@@ -615,11 +615,10 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(P
 // ------------------------------------------------------------ C2R pass along x
 // Rows of the half spectrum X[0..n/2] (n = npx = 2^(L+1)) -> real row of n, cropped to nx,
 // with FP64 statistics.  Thread tid: row q = tid / G, butterfly slot g = tid % G.
-// PR: rows per block (0: as many as make 256 threads; the fused launch asks for its tiles' count)
-template <int L, int RM, int PR = 0>
+template <int L, int RM>
 struct C2RCfg {
     static constexpr int N = 1 << L, EL = eptlog(L, RM), EPT = 1 << EL, G = N / EPT;
-    static constexpr int P = PR ? PR : (G >= 256 ? 1 : 256 / G), NT = G * P, M = npass(L, RM);
+    static constexpr int P = G >= 256 ? 1 : 256 / G, NT = G * P, M = npass(L, RM);
     static constexpr int ROW = N + (N >> EL);
     static constexpr int ppos(int pos) { return pos + (pos >> EL); }
     static constexpr size_t lds_elems() { return (size_t)ROW * P; }
@@ -658,9 +657,9 @@ __device__ __forceinline__ void c2r_operands(const f2* xa, const f2* xb, TW tw2,
 
 // The radix passes of the rows' transforms, exchanging through LDS between passes (the block
 // waits at a barrier before each exchange after the first, and after each write).
-template <int L, int RM, int PR, typename TW>
+template <int L, int RM, typename TW>
 __device__ __forceinline__ void c2r_passes(f2* v, f2* lds, const int q, const int g, TW tw) {
-    using C = C2RCfg<L, RM, PR>;
+    using C = C2RCfg<L, RM>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M;
     [&]<int... J>(std::integer_sequence<int, J...>) {
         (
@@ -689,8 +688,8 @@ __device__ __forceinline__ void c2r_passes(f2* v, f2* lds, const int q, const in
 }
 
 // Block statistics (sum, sum of squares; max|Im| = 0 for a real-output transform): wave sums
-// into red[0 .. 2 NW) (the exchange buffer, once every wave is past its last LDS read: no static
-// LDS, so the fused launch keeps two blocks per CU); on return every thread may read them.
+// into red[0 .. 2 NW) (the exchange buffer, once every wave is past its last LDS read); on return
+// every thread may read them.
 template <int NW>
 __device__ __forceinline__ void c2r_wave_sums(double s1, double s2, double* red) {
     s1 = wsum(s1);
@@ -705,9 +704,9 @@ __device__ __forceinline__ void c2r_wave_sums(double s1, double s2, double* red)
 }
 
 // One block of the C2R pass: rows rblock*P .. +P-1 of batch member b.
-template <int L, int IN, int RM, int PR = 0>
+template <int L, int IN, int RM>
 __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, const int b) {
-    using C = C2RCfg<L, RM, PR>;
+    using C = C2RCfg<L, RM>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, P = C::P, M = C::M;
     f2* lds = reinterpret_cast<f2*>(fast_smem);
     const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);    // W_N table
@@ -759,7 +758,7 @@ __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, co
         for (int r = 0; r < EPT; ++r) v[r] = mk(0.f, 0.f);
     }
 
-    c2r_passes<L, RM, PR>(v, lds, q, g, TwGlobal{tw});
+    c2r_passes<L, RM>(v, lds, q, g, TwGlobal{tw});
 
     double s1 = 0.0, s2 = 0.0;
     if (active) {
@@ -794,29 +793,20 @@ __device__ __forceinline__ void c2r_tile(const PassArgs& a, const int rblock, co
                 }
             }
     }
-    // A block of PR rows writes the partials of the standard blocks it spans (their waves, in
-    // their order), so that the statistics do not depend on the launch that ran the pass;
-    // a.nblocks counts standard blocks.
     constexpr int NW = (C::NT + 63) / 64;
-    constexpr int SW = (C2RCfg<L, RM>::NT + 63) / 64, NSUB = NW / SW;
-    static_assert(NW % SW == 0 && C::P % C2RCfg<L, RM>::P == 0, "whole standard blocks");
     static_assert(C::lds_elems() * sizeof(f2) >= 2 * NW * sizeof(double), "room for the block sums");
     double* red = reinterpret_cast<double*>(fast_smem);
     c2r_wave_sums<NW>(s1, s2, red);
     if (tid == 0) {
-        for (int sb = 0; sb < NSUB; ++sb) {
-            const int blk = rblock * NSUB + sb;
-            if (blk >= a.nblocks) break;
-            double t1 = 0, t2 = 0;
-            for (int w = sb * SW; w < (sb + 1) * SW; ++w) {
-                t1 += red[w];
-                t2 += red[NW + w];
-            }
-            double* p = a.partial + 3 * ((int64_t)b * a.nblocks + blk);
-            p[0] = t1;
-            p[1] = t2;
-            p[2] = 0.0;
+        double t1 = 0, t2 = 0;
+        for (int w = 0; w < NW; ++w) {
+            t1 += red[w];
+            t2 += red[NW + w];
         }
+        double* p = a.partial + 3 * ((int64_t)b * a.nblocks + rblock);
+        p[0] = t1;
+        p[1] = t2;
+        p[2] = 0.0;
     }
 }
 
@@ -834,6 +824,9 @@ __device__ __forceinline__ void bstore8(f2 v, __amdgpu_buffer_rsrc_t rs, uint32_
     const unsigned int __attribute__((ext_vector_type(2))) w = {__float_as_uint(v.x), __float_as_uint(v.y)};
     __builtin_amdgcn_raw_buffer_store_b64(w, rs, off, 0, 2);
 }
+__device__ __forceinline__ void bstore4(float v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, off, 0, 2);
+}
 __device__ __forceinline__ void bstore8d(double v, __amdgpu_buffer_rsrc_t rs, uint32_t off) {
     const uint64_t u = __double_as_longlong(v);
     const unsigned int __attribute__((ext_vector_type(2))) w = {(unsigned int)u, (unsigned int)(u >> 32)};
@@ -841,27 +834,25 @@ __device__ __forceinline__ void bstore8d(double v, __amdgpu_buffer_rsrc_t rs, ui
 }
 
 // The C2R pass from the y output (IN_LOADH) over members b0 .. b1-1 of one tile of P rows (rows in
-// one wave, an even cropped length, 8-byte aligned output rows): the rows of member b+1 are loaded
-// while member b is transformed and stored.  Every access goes through a buffer resource with
-// out-of-range offsets for rows past the end, members past the last and the lanes that write no
-// partial, so every wave issues the same memory instructions per member and the wait for the
-// prefetched rows never includes the stores of the member before.  Both twiddle tables are copied
-// to LDS behind the exchange buffer (a load from memory would complete only after the prefetch).
-// Used for such passes whatever the members per block (one included) and inside the three-role
-// launch, so that a realization's values never depend on the schedule.  A tile of PR rows writes
-// the partials of the standard tiles it spans, as c2r_tile does.
-template <int L, int RM, int PR = 0>
+// one wave): the rows of member b+1 are loaded while member b is transformed and stored.  EVEN (an
+// even cropped length and row and member strides, 8-byte aligned output): output pairs go out as
+// one 8-byte store; otherwise as two 4-byte stores, the second dropped past an odd length.  Every
+// access goes through a buffer resource with out-of-range offsets for rows past the end, members
+// past the last and the lanes that write no partial, so every wave issues the same memory
+// instructions per member and the wait for the prefetched rows never includes the stores of the
+// member before.  Both twiddle tables are copied to LDS behind the exchange buffer (a load from
+// memory would complete only after the prefetch).  Per member, the arithmetic and the statistics
+// partials are those of c2r_tile.
+template <int L, int RM>
 struct C2RRowsCfg {
-    using C = C2RCfg<L, RM, PR>;
+    using C = C2RCfg<L, RM>;
     static constexpr size_t lds_bytes() { return (C::lds_elems() + 2 * (size_t)C::N) * sizeof(f2); }
 };
-template <int L, int RM, int PR>
+template <int L, int RM, bool EVEN>
 __device__ __forceinline__ void c2r_rows(const PassArgs& a, const int rblock, const int b0, const int b1) {
-    using C = C2RCfg<L, RM, PR>;
+    using C = C2RCfg<L, RM>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, P = C::P, M = C::M, NW = (C::NT + 63) / 64;
-    constexpr int SW = (C2RCfg<L, RM>::NT + 63) / 64, NSUB = NW / SW;
     static_assert(G <= 64 && M > 0, "rows inside one wave, at least one radix pass");
-    static_assert(NW % SW == 0 && C::P % C2RCfg<L, RM>::P == 0, "whole standard tiles");
     static_assert(C::lds_elems() * sizeof(f2) >= 2 * NW * sizeof(double), "room for the block sums");
     f2* lds = reinterpret_cast<f2*>(fast_smem);
     f2* twl = lds + C::lds_elems();  // W_N (N entries), then the first N entries of W_{2N}
@@ -883,14 +874,18 @@ __device__ __forceinline__ void c2r_rows(const PassArgs& a, const int rblock, co
         for (int r = 0; r < EPT; ++r) nxt[r] = bload8(rs, ok ? (ibase + r * G) * 8u : kOob);
     };
     constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
-    __syncthreads();  // twiddles in LDS (and, in a fused launch, every wave past its role's setup)
+    __syncthreads();  // twiddles in LDS
     load(b0);
     {
         // as many out-of-range (dropped) stores as a member issues: the loop is entered with the
         // same vector-memory counter history as from its back edge
         const __amdgpu_buffer_rsrc_t rs = rsrc(a.out);
 #pragma unroll
-        for (int e = 0; e < EPT + 3 * NSUB; ++e) bstore8(mk(0.f, 0.f), rs, kOob + 8u * e);
+        for (int e = 0; e < EPT + 3; ++e) bstore8(mk(0.f, 0.f), rs, kOob + 8u * e);
+        if constexpr (!EVEN) {
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) bstore4(0.f, rs, kOob + 4u * e);
+        }
     }
     const uint32_t obase = (uint32_t)q * (uint32_t)a.out_row;
     const int nx = a.n_out;
@@ -904,7 +899,7 @@ __device__ __forceinline__ void c2r_rows(const PassArgs& a, const int rblock, co
         // column 0 = g0 + i gN (packed Nyquist column): X[0] = Re, X[N] = Im
         if (g == 0) xa[0].y = 0.f;
         c2r_operands<EPT, G>(xa, xb, TwLds{twl + N}, g, v);
-        c2r_passes<L, RM, PR>(v, lds, q, g, TwLds{twl});
+        c2r_passes<L, RM>(v, lds, q, g, TwLds{twl});
         double s1 = 0.0, s2 = 0.0;
         const __amdgpu_buffer_rsrc_t ro =
             rsrc(reinterpret_cast<float*>(a.out) + (int64_t)b * a.out_bstride + (int64_t)rblock * P * a.out_row);
@@ -915,69 +910,41 @@ __device__ __forceinline__ void c2r_rows(const PassArgs& a, const int rblock, co
                 const int m = out_pos<L, M - 1, RM>(g + G * i, r);
                 const f2 z = v[i * (1 << LRl) + r];
                 const bool ok = active && 2 * m + 1 < nx;
-                bstore8(z, ro, ok ? (obase + 2 * m) * 4u : kOob);
+                if constexpr (EVEN) {
+                    bstore8(z, ro, ok ? (obase + 2 * m) * 4u : kOob);
+                } else {
+                    const bool okx = active && 2 * m < nx;
+                    bstore4(z.x, ro, okx ? (obase + 2 * m) * 4u : kOob);
+                    bstore4(z.y, ro, ok ? (obase + 2 * m + 1) * 4u : kOob);
+                }
                 if (ok) {
                     s1 += (double)z.x + (double)z.y;
                     s2 += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+                } else if (!EVEN && active && 2 * m < nx) {
+                    s1 += (double)z.x;
+                    s2 += (double)z.x * (double)z.x;
                 }
             }
         double* red = reinterpret_cast<double*>(fast_smem);
         c2r_wave_sums<NW>(s1, s2, red);
-#pragma unroll
-        for (int sb = 0; sb < NSUB; ++sb) {
-            const int blk = rblock * NSUB + sb;
-            double t1 = 0, t2 = 0;
-            for (int w = sb * SW; w < (sb + 1) * SW; ++w) {
-                t1 += red[w];
-                t2 += red[NW + w];
-            }
-            const __amdgpu_buffer_rsrc_t rp = rsrc(a.partial + 3 * ((int64_t)b * a.nblocks + min(blk, a.nblocks - 1)));
-            const uint32_t po = tid == 0 && blk < a.nblocks ? 0u : kOob;
-            bstore8d(t1, rp, po);
-            bstore8d(t2, rp, po + 8u);
-            bstore8d(0.0, rp, po + 16u);
+        double t1 = 0, t2 = 0;
+        for (int w = 0; w < NW; ++w) {
+            t1 += red[w];
+            t2 += red[NW + w];
         }
+        const __amdgpu_buffer_rsrc_t rp = rsrc(a.partial + 3 * ((int64_t)b * a.nblocks + rblock));
+        const uint32_t po = tid == 0 ? 0u : kOob;
+        bstore8d(t1, rp, po);
+        bstore8d(t2, rp, po + 8u);
+        bstore8d(0.0, rp, po + 16u);
     }
 }
 
 // The C2R pass from the y output over a.bpb members per block (grid y: member groups).
-template <int L, int RM>
+template <int L, int RM, bool EVEN>
 __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_loop_kernel(PassArgs a) {
     const int b0 = blockIdx.y * a.bpb;
-    c2r_rows<L, RM, 0>(a, blockIdx.x, b0, min(b0 + a.bpb, a.nbatch));
-}
-
-// ------------------------------------------------------------ fill, y and C2R passes in one launch
-// The fill of batch k, the y pass of batch k-1 and the C2R x pass of batch k-2 (the x role reads
-// the y output of two launches before, which is double-buffered).  Roles interleave in periods
-// of 8A fill, 8B y and 8C x blocks (A:B:C the ratio of their block counts), K times, then the
-// rest of each.  Each tile runs the code of its separate kernel (c2r_rows for the x role, over
-// one member), so every value is that of separate launches.  The C2R tile takes PX rows, as many
-// as give the fill's thread count.  (Measured no faster than the two-role launch followed by the
-// separate x pass: the launch is bound by instruction issue, and the x role adds its own.)
-template <int L, int P, int RM, int IG, int LY, int PY, int RMY, bool HALF, int LX, int RMX, int PX>
-__global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_x_kernel(PassArgs az, PassArgs ay, PassArgs ax,
-                                                                                 FuseMap3 m) {
-    static_assert(StridedCfg<LY, PY, RMY>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
-    static_assert(C2RCfg<LX, RMX, PX>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
-    const int per = 8 * (m.A + m.B + m.C), bid = blockIdx.x, full = m.K * per;
-    int role, l;
-    if (bid < full) {
-        const int pr = bid / per, o = bid - pr * per;
-        role = o < 8 * m.A ? 0 : (o < 8 * (m.A + m.B) ? 1 : 2);
-        l = role == 0 ? pr * 8 * m.A + o
-                      : (role == 1 ? pr * 8 * m.B + (o - 8 * m.A) : pr * 8 * m.C + (o - 8 * (m.A + m.B)));
-    } else {
-        const int o = bid - full, zr = m.nzb - 8 * m.A * m.K, yr = m.nyb - 8 * m.B * m.K;
-        role = o < zr ? 0 : (o < zr + yr ? 1 : 2);
-        l = role == 0 ? 8 * m.A * m.K + o : (role == 1 ? 8 * m.B * m.K + (o - zr) : 8 * m.C * m.K + (o - zr - yr));
-    }
-    if (role == 0)
-        strided_tile<L, P, IN_FILL32H, 2, RM, IG, HALF>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
-    else if (role == 1)
-        strided_tile<LY, PY, IN_LOAD, 1, RMY, 0, HALF>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
-    else
-        c2r_rows<LX, RMX, PX>(ax, l % m.nxr, l / m.nxr, l / m.nxr + 1);
+    c2r_rows<L, RM, EVEN>(a, blockIdx.x, b0, min(b0 + a.bpb, a.nbatch));
 }
 
 // 1-point x axis (npx = 1): real part of the only coefficient of each row.

@@ -115,125 +115,13 @@ hipError_t fused_rm(int L, int LY, int nblocks, const PassArgs& az, const PassAr
                      : fused_dispatch<RM, 0>(L, LY, nblocks, az, ay, m, s, seq{});
 }
 
-// the member-loop C2R code (c2r_rows) serves a pass from the y output whose buffer offsets fit
-// it: an even cropped length, even output rows, 8-byte aligned output, tiles of P rows within 2 GiB
+// the member-loop C2R code (c2r_rows) serves a pass from the y output whose tiles of P rows span
+// less than 2 GiB (buffer offsets); output pairs as 8-byte stores where c2r_rows_even holds
 inline bool c2r_rows_fit(const PassArgs& a, int P) {
-    return ((a.n_out | a.out_row | a.out_bstride) & 1) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 7) == 0 &&
-           (int64_t)P * a.in_row * 8 < 0x7fffffffLL && (int64_t)P * a.out_row * 4 < 0x7fffffffLL;
+    return (int64_t)P * a.in_row * 8 < 0x7fffffffLL && (int64_t)P * a.out_row * 4 < 0x7fffffffLL;
 }
-
-// ---- fill, y and C2R x passes in one launch (fill_load_x_kernel).  Instantiated for fills of
-// length 2^kFuse3Log with plan 3 or 4 whose z and y passes keep at most half their axes (HALF), a
-// y pass of the fill's length and plan or of twice its length with plan 3, and a C2R pass of
-// complex length 2^(kFuse3Log-1) or 2^kFuse3Log with plan 3 or 4: the 256^3-in-512^3 and
-// 512x512x256-in-1024x1024x512 shapes.  The C2R tile takes as many rows as give the fill's
-// thread count; two blocks of the largest role must fit the CU's LDS.
-constexpr int kFuse3Log = 9;
-
-template <int LX, int RMX, int NT>
-constexpr int fused3_rows() {
-    using C = fast::C2RCfg<LX, RMX>;
-    return NT % C::G == 0 ? NT / C::G : 0;
-}
-
-template <int L, int RM, int LY, int RMY, int LX, int RMX>
-constexpr bool fused3_ok() {
-    if constexpr (RM < 3 || RM > 4 || RMX < 3 || RMX > 4 || (LY != L && LY != L + 1)) {
-        return false;
-    } else {
-        using C = fast::StridedCfg<L, strided_P(L), RM>;
-        using CY = fast::StridedCfg<LY, strided_P(LY), RMY>;
-        constexpr int PX = fused3_rows<LX, RMX, C::NT>();
-        if constexpr (PX == 0) {
-            return false;
-        } else {
-            using CX = fast::C2RCfg<LX, RMX, PX>;
-            constexpr size_t lx = fast::C2RRowsCfg<LX, RMX, PX>::lds_bytes();
-            constexpr size_t lds = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
-            constexpr bool ypair = LY == L ? RMY == RM : (RMY == 3 && fused_up<L, RM>());
-            return ypair && CX::G <= 64 && CY::NT == C::NT && CX::NT == C::NT && 2 * (lds > lx ? lds : lx) <= 163840;
-        }
-    }
-}
-
-template <int L, int RM, int IG, int LY, int RMY, int LX, int RMX>
-hipError_t fused3_L(int nblocks, const PassArgs& az, const PassArgs& ay, const PassArgs& ax, const FuseMap3& m,
-                    hipStream_t s) {
-    if constexpr (!fused3_ok<L, RM, LY, RMY, LX, RMX>()) {
-        return hipErrorInvalidValue;
-    } else {
-        using C = fast::StridedCfg<L, strided_P(L), RM>;
-        using CY = fast::StridedCfg<LY, strided_P(LY), RMY>;
-        constexpr int PX = fused3_rows<LX, RMX, C::NT>();
-        static_assert(3 * sizeof(PassArgs) + sizeof(FuseMap3) <= 4096, "kernel arguments");
-        static bool attr = false;
-        const auto k = fast::fill_load_x_kernel<L, strided_P(L), RM, IG, LY, strided_P(LY), RMY, true, LX, RMX, PX>;
-        constexpr size_t lx = fast::C2RRowsCfg<LX, RMX, PX>::lds_bytes();
-        constexpr size_t lzy = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
-        constexpr size_t lds = lzy > lx ? lzy : lx;
-        if (lds > 65536 && !attr) {
-            hipError_t e =
-                hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-            attr = true;
-        }
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(C::NT), lds, s, az, ay, ax, m);
-        return hipGetLastError();
-    }
-}
-
-// rows per C2R block of the three-role launch for these lengths and plans, 0 where none exists
-template <int RM>
-int fused3_query(int L, int LY, int rmy, int LX, int rmx) {
-    constexpr int L0 = kFuse3Log;
-    if (L != L0) return 0;
-    int px = 0;
-    auto one = [&]<int LYc, int RMYc, int LXc, int RMXc>() {
-        if constexpr (fused3_ok<L0, RM, LYc, RMYc, LXc, RMXc>()) {
-            if (LY == LYc && rmy == RMYc && LX == LXc && rmx == RMXc)
-                px = fused3_rows<LXc, RMXc, fast::StridedCfg<L0, strided_P(L0), RM>::NT>();
-        }
-    };
-    one.template operator()<L0, RM, L0 - 1, 3>();
-    one.template operator()<L0, RM, L0 - 1, 4>();
-    one.template operator()<L0, RM, L0, 3>();
-    one.template operator()<L0, RM, L0, 4>();
-    one.template operator()<L0 + 1, 3, L0 - 1, 3>();
-    one.template operator()<L0 + 1, 3, L0 - 1, 4>();
-    one.template operator()<L0 + 1, 3, L0, 3>();
-    one.template operator()<L0 + 1, 3, L0, 4>();
-    return px;
-}
-
-template <int RM, int IG>
-hipError_t fused3_ig(int LY, int LX, int rmx, int nblocks, const PassArgs& az, const PassArgs& ay, const PassArgs& ax,
-                     const FuseMap3& m, hipStream_t s) {
-    constexpr int L0 = kFuse3Log;
-    hipError_t e = hipErrorInvalidValue;
-    auto one = [&]<int LYc, int RMYc, int LXc, int RMXc>() {
-        if (LY == LYc && LX == LXc && rmx == RMXc) e = fused3_L<L0, RM, IG, LYc, RMYc, LXc, RMXc>(nblocks, az, ay, ax, m, s);
-    };
-    one.template operator()<L0, RM, L0 - 1, 3>();
-    one.template operator()<L0, RM, L0 - 1, 4>();
-    one.template operator()<L0, RM, L0, 3>();
-    one.template operator()<L0, RM, L0, 4>();
-    one.template operator()<L0 + 1, 3, L0 - 1, 3>();
-    one.template operator()<L0 + 1, 3, L0 - 1, 4>();
-    one.template operator()<L0 + 1, 3, L0, 3>();
-    one.template operator()<L0 + 1, 3, L0, 4>();
-    return e;
-}
-
-template <int RM>
-hipError_t fused3_rm(int L, int LY, int LX, int rmx, int nblocks, const PassArgs& az, const PassArgs& ay,
-                     const PassArgs& ax, const FuseMap3& m, hipStream_t s) {
-    if constexpr (RM < 3 || RM > 4) {
-        return hipErrorInvalidValue;
-    } else {
-        if (L != kFuse3Log) return hipErrorInvalidValue;
-        return az.igauss ? fused3_ig<RM, 1>(LY, LX, rmx, nblocks, az, ay, ax, m, s)
-                         : fused3_ig<RM, 0>(LY, LX, rmx, nblocks, az, ay, ax, m, s);
-    }
+inline bool c2r_rows_even(const PassArgs& a) {
+    return ((a.n_out | a.out_row | a.out_bstride) & 1) == 0 && (reinterpret_cast<uintptr_t>(a.out) & 7) == 0;
 }
 
 template <int L, int IN, int RM>
@@ -241,9 +129,13 @@ hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using C = fast::C2RCfg<L, RM>;
     if constexpr (IN == IN_LOADH && C::G <= 64 && C::M > 0) {
         if (c2r_rows_fit(a, C::P)) {
-            static bool attr_loop = false;
-            return launch_k(fast::c2r_loop_kernel<L, RM>, fast::C2RRowsCfg<L, RM>::lds_bytes(), C::NT, nblocks,
-                            (nbatch + a.bpb - 1) / a.bpb, a, s, attr_loop);
+            static bool attr_even = false, attr_odd = false;
+            const int groups = (nbatch + a.bpb - 1) / a.bpb;
+            if (c2r_rows_even(a))
+                return launch_k(fast::c2r_loop_kernel<L, RM, true>, fast::C2RRowsCfg<L, RM>::lds_bytes(), C::NT, nblocks,
+                                groups, a, s, attr_even);
+            return launch_k(fast::c2r_loop_kernel<L, RM, false>, fast::C2RRowsCfg<L, RM>::lds_bytes(), C::NT, nblocks,
+                            groups, a, s, attr_odd);
         }
     }
     static bool attr = false;

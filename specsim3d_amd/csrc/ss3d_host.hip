@@ -321,8 +321,7 @@ struct ss3d_context {
     void* scratch = nullptr;
     Lane sl;
     bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
-    int fuse = 1;            // SS3D_OPT_FUSE: 1 fill of batch k+1 and y pass of batch k in one launch, 2 also
-                             // the C2R x pass of batch k-1 in it, 0 separate launches
+    bool fuse = true;        // SS3D_OPT_FUSE: fill of batch k+1 and y pass of batch k in one launch
     bool batch_tune = true;  // SS3D_OPT_BATCH_AUTOTUNE: batch creation autotunes every distinct grid
     bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
     int target_e64 = 4096;   // elements per FP64 tile
@@ -642,45 +641,6 @@ int run_fused(ss3d_context* c, Pass& pz, int nbz, Pass& py, int nby, hipStream_t
     if (c->profiling) {
         HIP_TRY(hipEventRecord(e1, strm));
         c->prof.push_back({9, e0, e1, pass_bytes(pz, nbz, gz) + pass_bytes(py, nby, gy)});
-    }
-    return SS3D_OK;
-}
-
-// The fill pass pz over nbz members, the y pass py over nby members and the C2R x pass px over nbx
-// members in one launch of the three-role kernel (pass kind 10); the x role's tiles take `rows`
-// rows.  px.a.nblocks stays the count of standard C2R blocks (statistics partials, finalize).
-int run_fused3(ss3d_context* c, Pass& pz, int nbz, Pass& py, int nby, Pass& px, int nbx, int rows, hipStream_t strm,
-               DevBuf& partial) {
-    int gz = 1, gy = 1, gx = 1;
-    SS3D_TRY(prep_pass(pz, nbz, strm, partial, gz));
-    SS3D_TRY(prep_pass(py, nby, strm, partial, gy));
-    SS3D_TRY(prep_pass(px, nbx, strm, partial, gx));
-    const int64_t nxr = (px.a.nrows + rows - 1) / rows;
-    const int64_t nzb = (int64_t)pz.nblocks * gz, nyb = (int64_t)py.nblocks * gy, nxb = nxr * nbx;
-    if (nzb + nyb + nxb > 0x7fffffffLL) return fail(SS3D_ETOOBIG, "fused pass of %lld blocks", (long long)(nzb + nyb + nxb));
-    FuseMap3 m;
-    m.nzb = (int)nzb;
-    m.nyb = (int)nyb;
-    m.nxb = (int)nxb;
-    m.nxr = (int)nxr;
-    const int64_t mn = std::max<int64_t>(1, std::min({nzb, nyb, nxb}));
-    m.A = (int)std::max<int64_t>(1, (nzb + mn / 2) / mn);
-    m.B = (int)std::max<int64_t>(1, (nyb + mn / 2) / mn);
-    m.C = (int)std::max<int64_t>(1, (nxb + mn / 2) / mn);
-    m.K = std::min({m.nzb / (8 * m.A), m.nyb / (8 * m.B), m.nxb / (8 * m.C)});
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->profiling) {
-        e0 = take_event(c);
-        e1 = take_event(c);
-        if (!e0 || !e1) return fail(SS3D_EHIP, "hipEventCreate failed");
-        HIP_TRY(hipEventRecord(e0, strm));
-    }
-    hipError_t e = launch_fast_fused3(pz.a.logn, pz.rm, py.a.logn, px.a.logn, px.rm, (int)(nzb + nyb + nxb), pz.a, py.a,
-                                      px.a, m, strm);
-    if (e != hipSuccess) return fail(SS3D_EHIP, "fused pass launch (3 roles): %s", hipGetErrorString(e));
-    if (c->profiling) {
-        HIP_TRY(hipEventRecord(e1, strm));
-        c->prof.push_back({10, e0, e1, pass_bytes(pz, nbz, gz) + pass_bytes(py, nby, gy) + pass_bytes(px, nbx, gx)});
     }
     return SS3D_OK;
 }
@@ -1178,7 +1138,7 @@ Fp32Work fp32_work(const ss3d_spectrum* sp, int nx, int ny, int nz, int ctile) {
 // nb_cap > 0 bounds the realizations per launch (ss3d_realize_device_noalloc: what its scratch
 // windows hold).
 int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int igauss, int64_t r0, int nreal, int nx, int ny,
-                 int nz, void* dout, double* dstats, int nb_cap = 0, bool allow_fuse3 = true) {
+                 int nz, void* dout, double* dstats, int nb_cap = 0) {
     const int npx = sp->npx, npy = sp->npy, npz = sp->npz;
     const int lx = ilog2(npx), ly = ilog2(npy), lz = ilog2(npz);
     if (lx > kMaxFastLog || ly > kMaxFastLog || lz > kMaxFastLog)
@@ -1277,57 +1237,8 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     // fusion keeps the y pass's own radix plan (results do not depend on the schedule): the fill's
     // plan at the same length, plan 3 at twice the length
     const int rmy_fused = lz == ly ? pz.rm : 3;
-    const bool fuse2 = staged && has_z && has_y && !pipelined && c->fuse >= 1 && nbat >= 2 && pz.var == 0 &&
+    const bool fuse2 = staged && has_z && has_y && !pipelined && c->fuse && nbat >= 2 && pz.var == 0 &&
                        py.var == 0 && fast_same_plan(ly, py.rm, rmy_fused) && fast_fusable(lz, pz.rm, ly, rmy_fused);
-    // the C2R x pass as a third role: its plan has a three-role instance beside these two, and its
-    // output layout is one the separate pass runs through the same member-loop code (even rows
-    // and members, 8-byte aligned)
-    const bool x_rows = ((nx | ncrop) & 1) == 0 && (reinterpret_cast<uintptr_t>(dout) & 7) == 0;
-    const int rows3 = fuse2 && allow_fuse3 && c->fuse >= 2 && nbat >= 3 && px.fast == 2 && px.in == IN_LOADH && x_rows
-                          ? fast_fused3_rows(lz, pz.rm, ly, rmy_fused, px.a.logn, px.rm)
-                          : 0;
-    if (rows3 > 0) {
-        // launch k: fill of batch k, y pass of batch k-1, C2R x pass of batch k-2.  The y output
-        // alternates between two buffers (wb, wa) so that the x role reads one while the y role
-        // writes the other; the first two and last two launches hold fewer roles.
-        py.rm = rmy_fused;
-        SS3D_TRY(ensure2(L, L.wa, (size_t)wk.yout * nb * 8));
-        DevBuf* yb[2] = {&L.wb, &L.wa};
-        for (int k = 0; k <= nbat + 1; ++k) {
-            const bool zk = k < nbat, yk = k >= 1 && k <= nbat, xk = k >= 2;
-            const int cz = zk ? std::min(nb, nreal - k * nb) : 0;
-            const int y0 = (k - 1) * nb, cy = yk ? std::min(nb, nreal - y0) : 0;
-            const int x0 = (k - 2) * nb, cx = xk ? std::min(nb, nreal - x0) : 0;
-            if (zk) {
-                pz.a.rbase = L.rbase.as<uint32_t>() + (size_t)k * nb;
-                pz.a.out = L.pipe[k & 1].p;
-            }
-            if (yk) {
-                py.a.in = L.pipe[(k - 1) & 1].p;
-                py.a.out = yb[(k - 1) & 1]->p;
-            }
-            if (xk) {
-                px.a.in = yb[(k - 2) & 1]->p;
-                px.a.out = static_cast<char*>(dout) + (size_t)x0 * ncrop * 4;
-            }
-            if (zk && yk && xk) {
-                SS3D_TRY(run_fused3(c, pz, cz, py, cy, px, cx, rows3, L.stream, L.partial));
-            } else {
-                if (zk && yk)
-                    SS3D_TRY(run_fused(c, pz, cz, py, cy, L.stream, L.partial));
-                else if (zk)
-                    SS3D_TRY(run_pass(c, pz, cz, L.stream, L.partial));
-                else if (yk)
-                    SS3D_TRY(run_pass(c, py, cy, L.stream, L.partial));
-                if (xk) SS3D_TRY(run_pass(c, px, cx, L.stream, L.partial));
-            }
-            if (!xk) continue;
-            hipError_t e = launch_stats_finalize(L.partial.as<double>(), px.nblocks, cx, (double)ncrop,
-                                                 dstats + 3 * (size_t)x0, L.stream);
-            if (e != hipSuccess) return fail(SS3D_EHIP, "stats finalize: %s", hipGetErrorString(e));
-        }
-        return SS3D_OK;
-    }
     if (fuse2) {
         // launch k: fill of batch k beside the y pass of batch k-1 (pipe buffers alternate),
         // then the x pass of batch k-1
@@ -1581,7 +1492,7 @@ int ss3d_realize_device_noalloc(ss3d_context* c, const ss3d_spectrum* spc, int i
         char* out = static_cast<char*>(dev_out) + (size_t)done * ncrop * esz;
         double* st = dev_stats ? dev_stats + 3 * (size_t)done : L.stats.as<double>();
         if (prec == SS3D_FP32) {
-            SS3D_TRY(enqueue_fp32(c, L, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, out, st, q.nb, false));
+            SS3D_TRY(enqueue_fp32(c, L, sp, iseed, igauss, r0 + done, cnt, nx, ny, nz, out, st, q.nb));
         } else {
             for (int k = 0; k < cnt; k += q.nb) {
                 const int m = std::min(q.nb, cnt - k);
@@ -1696,21 +1607,6 @@ int ss3d_autotune(ss3d_context* c, const ss3d_spectrum* spc, int nx, int ny, int
             pick[1] = py;
         }
     }
-    // with SS3D_OPT_FUSE 2 the C2R x pass joins the fused launch where its plan has a three-role
-    // instance beside the fill and y plans picked: prefer such a plan
-    if (c->fuse >= 2 && sp->npx >= 2 && sp->npy > 1 && sp->npz > 1 && fusable(pick[0], pick[1])) {
-        const int rmyf = lz == ly ? pick[0] : 3;
-        if (!fast_fused3_rows(lz, pick[0], ly, rmyf, Lk[2], pick[2])) {
-            double b = 1e30;
-            int bx = 0;
-            for (auto& kv : t[2])
-                if (fast_fused3_rows(lz, pick[0], ly, rmyf, Lk[2], kv.first) && kv.second < b) {
-                    b = kv.second;
-                    bx = kv.first;
-                }
-            if (bx) pick[2] = bx;
-        }
-    }
     for (int k = 0; k < 3; ++k) c->tune[{6 + k, Lk[k]}] = pick[k];
     return SS3D_OK;
 }
@@ -1728,8 +1624,7 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
         return SS3D_OK;
     }
     if (option == SS3D_OPT_FUSE) {
-        if (value < 0 || value > 2) return fail(SS3D_EINVAL, "SS3D_OPT_FUSE takes 0, 1 or 2");
-        c->fuse = value;
+        c->fuse = value != 0;
         return SS3D_OK;
     }
     if (option == SS3D_OPT_BATCH_AUTOTUNE) {

@@ -111,21 +111,10 @@ struct FuseMap {
     int nzb, nyb;  // blocks of the fill and of the y pass (tiles x member groups)
     int A, C, K;
 };
-// The same for the fill, y and C2R x passes in one launch (fill_load_x_kernel).
-struct FuseMap3 {
-    int nzb, nyb, nxb;  // blocks of the fill, the y pass and the x pass
-    int nxr;            // x-pass blocks per batch member (rows of the fused tile each)
-    int A, B, C, K;
-};
 bool fast_same_plan(int L, int rm1, int rm2);  // the FP32 radix plans of length 2^L for rm1 and rm2 are one
 // a fused launch exists for a fill of length 2^L (plan rm) and a y pass of length 2^LY (plan rmy):
 // LY = L with rmy = rm, or LY = L + 1 with rmy = 3 where the tiles have one thread count
 bool fast_fusable(int L, int rm, int LY, int rmy);
-// rows per C2R block of a launch holding the fill (length 2^L, plan rm), the y pass (2^LY, plan
-// rmy) and the C2R x pass (complex length 2^LX, plan rmx), 0 where no such launch exists
-int fast_fused3_rows(int L, int rm, int LY, int rmy, int LX, int rmx);
-hipError_t launch_fast_fused3(int L, int rm, int LY, int LX, int rmx, int nblocks, const PassArgs& az,
-                              const PassArgs& ay, const PassArgs& ax, const FuseMap3& m, hipStream_t s);
 hipError_t launch_fast_fused(int L, int rm, int LY, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
                              hipStream_t s);
 

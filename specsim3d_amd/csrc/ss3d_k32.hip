@@ -9,12 +9,6 @@ hipError_t launch_fast_strided_r5(int, int, int, int, int, int, const PassArgs&,
 hipError_t launch_fast_fused_r3(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
 hipError_t launch_fast_fused_r4(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
 hipError_t launch_fast_fused_r5(int, int, const PassArgs&, const PassArgs&, const FuseMap&, hipStream_t);
-hipError_t launch_fast_fused3_r3(int, int, int, int, int, const PassArgs&, const PassArgs&, const PassArgs&,
-                                 const FuseMap3&, hipStream_t);
-hipError_t launch_fast_fused3_r4(int, int, int, int, int, const PassArgs&, const PassArgs&, const PassArgs&,
-                                 const FuseMap3&, hipStream_t);
-int fast_fused3_rows_r3(int, int, int, int, int);
-int fast_fused3_rows_r4(int, int, int, int, int);
 hipError_t launch_fast_c2r_r3(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r4(int, int, int, int, const PassArgs&, hipStream_t);
 hipError_t launch_fast_c2r_r5(int, int, int, int, const PassArgs&, hipStream_t);
@@ -79,23 +73,6 @@ hipError_t launch_fast_fused(int L, int rm, int LY, const PassArgs& az, const Pa
         case 3: return launch_fast_fused_r3(L, LY, az, ay, m, s);
         case 4: return launch_fast_fused_r4(L, LY, az, ay, m, s);
         case 5: return launch_fast_fused_r5(L, LY, az, ay, m, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-int fast_fused3_rows(int L, int rm, int LY, int rmy, int LX, int rmx) {
-    switch (rm) {
-        case 3: return fast_fused3_rows_r3(L, LY, rmy, LX, rmx);
-        case 4: return fast_fused3_rows_r4(L, LY, rmy, LX, rmx);
-        default: return 0;
-    }
-}
-
-hipError_t launch_fast_fused3(int L, int rm, int LY, int LX, int rmx, int nblocks, const PassArgs& az,
-                              const PassArgs& ay, const PassArgs& ax, const FuseMap3& m, hipStream_t s) {
-    switch (rm) {
-        case 3: return launch_fast_fused3_r3(L, LY, LX, rmx, nblocks, az, ay, ax, m, s);
-        case 4: return launch_fast_fused3_r4(L, LY, LX, rmx, nblocks, az, ay, ax, m, s);
         default: return hipErrorInvalidValue;
     }
 }

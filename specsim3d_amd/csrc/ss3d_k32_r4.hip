@@ -9,12 +9,4 @@ hipError_t launch_fast_strided_r4(int L, int in_mode, int axis, int var, int nbl
 hipError_t launch_fast_fused_r4(int L, int LY, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     return fastlaunch::fused_rm<4>(L, LY, m.nzb + m.nyb, az, ay, m, s);
 }
-hipError_t launch_fast_c2r_r4(int L, int in_mode, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
-    return fastlaunch::c2r_rm<4>(L, in_mode, nblocks, nbatch, a, s);
-}
-hipError_t launch_fast_fused3_r4(int L, int LY, int LX, int rmx, int nblocks, const PassArgs& az, const PassArgs& ay,
-                                  const PassArgs& ax, const FuseMap3& m, hipStream_t s) {
-    return fastlaunch::fused3_rm<4>(L, LY, LX, rmx, nblocks, az, ay, ax, m, s);
-}
-int fast_fused3_rows_r4(int L, int LY, int rmy, int LX, int rmx) { return fastlaunch::fused3_query<4>(L, LY, rmy, LX, rmx); }
 }  // namespace ss3d

@@ -9,7 +9,4 @@ hipError_t launch_fast_strided_r5(int L, int in_mode, int axis, int var, int nbl
 hipError_t launch_fast_fused_r5(int L, int LY, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     return fastlaunch::fused_rm<5>(L, LY, m.nzb + m.nyb, az, ay, m, s);
 }
-hipError_t launch_fast_c2r_r5(int L, int in_mode, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
-    return fastlaunch::c2r_rm<5>(L, in_mode, nblocks, nbatch, a, s);
-}
 }  // namespace ss3d

@@ -1,15 +1,13 @@
-"""The fused FP32 schedules (SS3D_OPT_FUSE): with 1 the z fill pass of batch k+1 and the y pass of
-batch k run in one launch whose blocks interleave (the default); with 2 the launch also holds the
-C2R x pass of batch k-1 where a three-role instance exists (fill length 2^9).  A fused launch
-performs the arithmetic of the separate passes on the same tiles, so the fields and statistics
-equal those of separate launches bit for bit.  The cases exercise the block-role map: more fill
+"""The fused FP32 schedule (SS3D_OPT_FUSE, on by default): the z fill pass of batch k+1 and the y
+pass of batch k run in one launch whose blocks interleave.  The launch performs the arithmetic of
+the separate passes on the same tiles, so the fields and statistics equal those of separate
+launches bit for bit.  The cases exercise the block-role map: more fill
 blocks than y blocks and fewer (a short last batch), a tail of one role, lengths 2^5 .. 2^10, both
 coefficient laws, a y axis twice the fill's, the instance that prunes the upper half of the last
 radix pass (both passes keep at most half their axes) and the one that does not, a grid whose z
-and y lengths differ otherwise (no fusion: separate launches either way), and the three-role
-launch at 512^3 and 1024x1024x512 (y twice the fill's length).  The fused pass is what runs: pass
-kind 9 appears in the profile only with the option on, kind 10 only with 2 and a fill of length
-2^9 (and plans 3 or 4, at least three batches in one call).
+and y lengths differ otherwise (no fusion: separate launches either way), and the full-size
+512^3 and 1024x1024x512 periodic grids (y twice the fill's length).  The fused pass is what runs:
+pass kind 9 appears in the profile only with the option on.
 """
 import sys
 
@@ -53,23 +51,20 @@ def test_fused_equals_separate_launches(eng, sts, nugget, grid, nreal, igauss, f
     sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
     try:
         got = {}
-        for opt in (2, 1, 0):
+        for opt in (1, 0):
             eng.set_option(S.OPT_FUSE, opt)
             eng.profile(True)
             got[opt] = eng.realize(sp, 777, igauss, 5, nreal, nx, ny, nz, precision=S.FP32)
             eng.profile(False)
             kinds = eng.profile_read()
-            assert ("real_zy_fp32" in kinds) == (fused and opt >= 1), kinds
-            assert "real_zyx_fp32" not in kinds, kinds  # no fill of length 2^9 here
+            assert ("real_zy_fp32" in kinds) == (fused and opt == 1), kinds
     finally:
         eng.set_option(S.OPT_FUSE, 1)
         sp.close()
-    (f0, m0, v0, i0) = got[0]
-    for opt in (2, 1):
-        f1, m1, v1, i1 = got[opt]
-        assert np.array_equal(f1, f0)
-        assert np.array_equal(m1, m0) and np.array_equal(v1, v0) and np.array_equal(i1, i0)
-    assert np.all(np.isfinite(f0)) and np.all(v0 > 0)
+    (f1, m1, v1, i1), (f0, m0, v0, i0) = got[1], got[0]
+    assert np.array_equal(f1, f0)
+    assert np.array_equal(m1, m0) and np.array_equal(v1, v0) and np.array_equal(i1, i0)
+    assert np.all(np.isfinite(f1)) and np.all(v1 > 0)
 
 
 def test_fused_matches_fp64(eng):
@@ -101,19 +96,19 @@ def test_no_fusion_across_radix_plans(eng):
         eng.tuning_set(6, ly, 4)
         eng.tuning_set(7, ly, 3)
         got = {}
-        for opt in (2, 1, 0):
+        for opt in (1, 0):
             eng.set_option(S.OPT_FUSE, opt)
             eng.profile(True)
             got[opt] = eng.realize(sp, 31, igauss, 0, nreal, nx, ny, nz, precision=S.FP32)[0]
             eng.profile(False)
             kinds = eng.profile_read()
-            assert "real_zy_fp32" not in kinds and "real_zyx_fp32" not in kinds, kinds
+            assert "real_zy_fp32" not in kinds, kinds
     finally:
         eng.set_option(S.OPT_FUSE, 1)
         for k, v in old.items():
             eng.tuning_set(k, ly, v)
         sp.close()
-    assert np.array_equal(got[2], got[0]) and np.array_equal(got[1], got[0])
+    assert np.array_equal(got[1], got[0])
 
 
 def test_autotune_keeps_fusion(eng):
@@ -160,24 +155,21 @@ def test_fused_longer_y_pass(eng, sts, grid, nreal, rmz):
         eng.tuning_set(6, lz, rmz)
         eng.tuning_set(7, ly, 3)
         got = {}
-        for opt in (2, 1, 0):
+        for opt in (1, 0):
             eng.set_option(S.OPT_FUSE, opt)
             eng.profile(True)
             got[opt] = eng.realize(sp, 99, 1, 2, nreal, nx, ny, nz, precision=S.FP32)
             eng.profile(False)
             kinds = eng.profile_read()
-            assert ("real_zy_fp32" in kinds) == (opt >= 1), kinds
-            assert "real_zyx_fp32" not in kinds, kinds  # x lengths without a three-role instance
+            assert ("real_zy_fp32" in kinds) == (opt == 1), kinds
     finally:
         eng.set_option(S.OPT_FUSE, 1)
         eng.tuning_set(6, lz, old[6])
         eng.tuning_set(7, ly, old[7])
         sp.close()
-    (f0, m0, v0, _) = got[0]
-    for opt in (2, 1):
-        f1, m1, v1, _ = got[opt]
-        assert np.array_equal(f1, f0)
-        assert np.array_equal(m1, m0) and np.array_equal(v1, v0)
+    (f1, m1, v1, _), (f0, m0, v0, _) = got[1], got[0]
+    assert np.array_equal(f1, f0)
+    assert np.array_equal(m1, m0) and np.array_equal(v1, v0)
 
 
 def full_size_runs(e, sts, nugget, grid, nreal, opts, seed, plans):
@@ -223,71 +215,47 @@ def assert_same_as_separate(got):
 
 
 # (fill plan, x plan) at 512^3: the y pass takes the fill's plan
-PLANS_512 = [(4, 3), (3, 4)]
+PLANS_512 = [(4, 3), (3, 5)]
 
 
 @pytest.mark.parametrize("rmz,rmx", PLANS_512)
 def test_fused_full_size(rmz, rmx):
     """The 256^3 output grid in a 512^3 periodic grid (exponential 0.7 + nugget 0.3, ranges 80/40/10,
-    az 30, dip 15, igauss=1), 33 realizations in batches of 8, 8, 8, 8, 1: three-role launches
-    (fill of batch k, y pass of batch k-1, C2R x pass of batch k-2; the last with a fill of one
-    realization) and two-role launches give the fields and statistics of separate launches bit for
-    bit, with C2R tiles of 16 rows (plan 3) and 32 rows (plan 4); realizations that went through
-    the three-role launch (3, 12) and the last one match the FP64 path to 1e-5 relative RMS."""
+    az 30, dip 15, igauss=1), 33 realizations in one call (batches 8, 8, 8, 8, 1): the fused
+    launches (the last with a fill of one realization) give the fields and statistics of separate
+    launches bit for bit, and realizations whose fill or y pass ran fused (3, 12) and the last one
+    match the FP64 path to 1e-5 relative RMS."""
     sts = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]
     plans = {(6, 9): rmz, (7, 9): rmz, (8, 8): rmx}
     with S.Engine(0) as e:
-        sp, got, kinds = full_size_runs(e, sts, 0.3, (256, 256, 256), 33, (2, 1, 0), 69069, plans)
+        sp, got, kinds = full_size_runs(e, sts, 0.3, (256, 256, 256), 33, (1, 0), 69069, plans)
         try:
-            assert kinds[2].get("real_zyx_fp32", (0, 0, 0))[2] == 3, kinds[2]
-            assert "real_zy_fp32" in kinds[1] and "real_zyx_fp32" not in kinds[1], kinds[1]
-            assert "real_zy_fp32" not in kinds[0] and "real_zyx_fp32" not in kinds[0], kinds[0]
+            assert kinds[1].get("real_zy_fp32", (0, 0, 0))[2] == 4, kinds[1]
+            assert "real_zy_fp32" not in kinds[0], kinds[0]
             say("[fused] full size: fp64 of realizations 3, 12 and 32")
             f64 = {r: e.realize(sp, 69069, 1, r, 1, 256, 256, 256, precision=S.FP64)[0][0] for r in (3, 12, 32)}
         finally:
             sp.close()
     assert_same_as_separate(got)
     for r, ref in f64.items():
-        rel = np.sqrt(np.mean((got[2][0][r].astype(np.float64) - ref) ** 2) / np.mean(ref ** 2))
+        rel = np.sqrt(np.mean((got[1][0][r].astype(np.float64) - ref) ** 2) / np.mean(ref ** 2))
         say(f"realization {r}: relative RMS {rel:.2e}")
         assert rel < 1e-5
-    v = got[2][2]
+    v = got[1][2]
     assert 0.5 < v.min() and v.max() < 1.5
 
 
-def test_fused_three_roles_longer_axes():
+def test_fused_full_size_longer_y_axis():
     """512x512x256 in 1024x1024x512 (Gaussian sill 1, ranges 150/60/20, az 120, dip -20, igauss=1):
-    the three-role launch with a y pass twice the fill's length (2^10, plan 3, beside 2^9, plan 4)
-    and a C2R pass of complex length 2^9 (plan 4, tiles of 8 rows); 20 realizations (batches 8, 8,
-    4) give the fields and statistics of separate launches bit for bit."""
+    the fused launch with a y pass twice the fill's length (2^10, plan 3, beside 2^9, plan 4); 20
+    realizations (batches 8, 8, 4) give the fields and statistics of separate launches bit for bit."""
     sts = [(3, 1.0, 150.0, 60.0, 20.0, 120.0, -20.0, 0.0)]
     plans = {(6, 9): 4, (7, 10): 3, (8, 9): 4}
     with S.Engine(0) as e:
-        sp, got, kinds = full_size_runs(e, sts, 0.0, (512, 512, 256), 20, (2, 0), 4711, plans)
+        sp, got, kinds = full_size_runs(e, sts, 0.0, (512, 512, 256), 20, (1, 0), 4711, plans)
         sp.close()
-    assert kinds[2].get("real_zyx_fp32", (0, 0, 0))[2] == 1, kinds[2]
-    assert "real_zy_fp32" not in kinds[0] and "real_zyx_fp32" not in kinds[0], kinds[0]
+    assert kinds[1].get("real_zy_fp32", (0, 0, 0))[2] == 2, kinds[1]
+    assert "real_zy_fp32" not in kinds[0], kinds[0]
     assert_same_as_separate(got)
-    v = got[2][2]
+    v = got[1][2]
     assert 0.3 < v.min() and v.max() < 2.0
-
-
-def test_autotune_picks_three_roles():
-    """With SS3D_OPT_FUSE 2, the autotuner's plans at 512^3 have a three-role launch, and 24
-    realizations after it run through one."""
-    sts = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]
-    npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, 256, 256, 256, 1)
-    with S.Engine(0) as e:
-        sp = e.spectrum(sts, 0.3, 1.0, 1.0, 1.0, *npg)
-        try:
-            e.set_option(S.OPT_FUSE, 2)
-            plan = e.autotune(sp, 256, 256, 256, 1, reps=1)
-            say(f"[fused] autotune 512^3: {plan}")
-            with DevBuf(24 * 256 ** 3 * 4) as out:
-                e.profile(True)
-                e.realize_device(sp, 7, 1, 0, 24, 256, 256, 256, S.FP32, out.ptr)
-                e.profile(False)
-                kinds = e.profile_read()
-        finally:
-            sp.close()
-    assert "real_zyx_fp32" in kinds, (plan, kinds)

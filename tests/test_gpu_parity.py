@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import golden_cases as G
+from hipmem import DevBuf
 import oracle_ffi as O
 import specsim3d_amd as S
 
@@ -105,8 +106,8 @@ def test_realization_offsets_are_independent(eng, precision, igauss):
 
 # (output grid, realizations, C2R radix plan): 256^3 in 512^3 (8 members per block); 250x130x110
 # in 512x512x256: 14300 rows in tiles of 8 (plan 3: 6 members per block, then 2) or 16 rows (plan
-# 4: 3, 3 and 2 members), the last tile ragged
-C2R_LOOP = [((256, 256, 256), 8, 4), ((250, 130, 110), 8, 3), ((250, 130, 110), 8, 4)]
+# 4: 3, 3 and 2 members), the last tile ragged; 251x130x110: an odd row length (4-byte stores)
+C2R_LOOP = [((256, 256, 256), 8, 4), ((250, 130, 110), 8, 3), ((250, 130, 110), 8, 4), ((251, 130, 110), 8, 3)]
 
 
 @pytest.mark.parametrize("grid,nreal,rm", C2R_LOOP)
@@ -133,6 +134,40 @@ def test_c2r_member_loop_equals_single_members(eng, grid, nreal, rm):
     finally:
         eng.tuning_set(8, lx - 1, old)
         sp.close()
+
+
+@pytest.mark.parametrize("rm", [3, 5])
+def test_c2r_output_alignment_does_not_change_values(eng, rm):
+    """Realizations written to an output buffer that is only 4-byte aligned (4-byte stores of the
+    output pairs) equal those written to an 8-byte aligned one (8-byte stores), bit for bit, with
+    their statistics; 250x130x110 in 512x512x256, 8 realizations in one launch."""
+    sts = [(2, 0.7, 80, 40, 10, 30, 15, 0)]
+    nx, ny, nz, nreal = 250, 130, 110, 8
+    npg = S.periodic_grid(sts, 1, 1, 1, nx, ny, nz, 1)
+    lx = npg[0].bit_length() - 1
+    ncell = nx * ny * nz
+    say(f"[c2r align] plan {rm}")
+    old = eng.tuning_get(8, lx - 1)
+    sp = eng.spectrum(sts, 0.3, 1, 1, 1, *npg)
+    out = DevBuf(nreal * ncell * 4 + 8)
+    st = DevBuf(2 * nreal * 3 * 8)
+    try:
+        eng.tuning_set(8, lx - 1, rm)
+        got = []
+        for off in (0, 4):
+            eng.realize_device(sp, 69069, 1, 0, nreal, nx, ny, nz, S.FP32, out.ptr + off, st.ptr + len(got) * nreal * 24)
+            eng.synchronize()
+            raw = out.get(np.uint8, nreal * ncell * 4 + 8)
+            got.append(raw[off:off + nreal * ncell * 4].view(np.float32).copy())
+        stats = st.get(np.float64, 2 * nreal * 3).reshape(2, nreal, 3)
+    finally:
+        eng.tuning_set(8, lx - 1, old)
+        out.free()
+        st.free()
+        sp.close()
+    assert np.array_equal(got[0], got[1])
+    assert np.array_equal(stats[0], stats[1])
+    assert np.all(np.isfinite(got[0])) and 0.5 < stats[0, :, 1].min()
 
 
 def test_256cubed_full_size_fp32_vs_fp64(eng):
