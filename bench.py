@@ -155,6 +155,78 @@ def trace_summary(trace_csv, marker_csv, bench_json, command):
     return "\n".join(out) + "\n"
 
 
+# pass kind -> kernel, for the FP64 realization passes as well (hardware-counter summary)
+KIND_KERNELS_ALL = dict(KIND_KERNELS, real_x_fp64=r"f64::rows_pair_kernel<", real_y_fp64=r"f64::strided_kernel<\d+, 0>",
+                        real_z_fp64=r"f64::strided_kernel<\d+, 2>")
+
+
+def traffic_summary(fetch_csv, write_csv, rdreq_csv, bench_json, command, out_json):
+    """HBM bytes per launch of each realization pass from rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE
+    and optionally the TCC_EA0_RDREQ request-size counters; one counter set per run, KiB or counts per
+    dispatch) of `command`, each scaled by what the counter reads for the 1 GiB device-to-device copies
+    of --copy-mib 1024.  Writes `out_json` (read by load_traffic) and returns a markdown table."""
+    import csv
+
+    def per_dispatch(path):
+        d = {}
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                key = (int(r["Dispatch_Id"]), r["Kernel_Name"], int(r["Grid_Size"]))
+                d.setdefault(key, {})
+                d[key][r["Counter_Name"]] = d[key].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        return d
+
+    with open(bench_json) as fh:
+        line = json.loads([ln for ln in fh.read().splitlines() if ln.startswith("{")][-1])
+    runs = {"FETCH_SIZE": per_dispatch(fetch_csv), "WRITE_SIZE": per_dispatch(write_csv)}
+    rd = per_dispatch(rdreq_csv) if rdreq_csv else {}
+    copy_kib = 1024 * 1024  # one 1 GiB copy, in KiB
+    factor = {}
+    for name, d in runs.items():
+        # the 1 GiB copies: the largest copy dispatches (smaller copies of the run are left out)
+        vals = sorted((v[name] for (_, k, _), v in d.items() if "copyBuffer" in k and name in v), reverse=True)
+        vals = [x for x in vals if x >= 0.5 * vals[0]] if vals else []
+        factor[name] = copy_kib / (sum(vals) / len(vals)) if vals else 1.0
+    kinds = {}
+    for kind, pat in KIND_KERNELS_ALL.items():
+        p = re.compile(pat)
+        ks = {}
+        for name, d in runs.items():
+            for (_, k, grid), v in d.items():
+                if p.search(k):
+                    ks.setdefault((k, grid), {}).setdefault(name, []).append(v[name])
+        if not ks or kind not in line["passes"]:
+            continue
+        # the kernel and grid the pass runs most often (the FP64 y kernel also serves the spectrum once)
+        (k, grid), c = max(ks.items(), key=lambda kv: len(kv[1].get("FETCH_SIZE", [])))
+        f = sum(c["FETCH_SIZE"]) / len(c["FETCH_SIZE"])
+        w = sum(c["WRITE_SIZE"]) / len(c["WRITE_SIZE"])
+        rq = [v for (_, kk, gg), v in rd.items() if kk == k and gg == grid]
+        rbytes = None
+        if rq:
+            rbytes = sum(32 * v.get("TCC_EA0_RDREQ_32B_sum", 0) + 64 * v.get("TCC_EA0_RDREQ_64B_sum", 0) +
+                         128 * v.get("TCC_EA0_RDREQ_128B_sum", 0) for v in rq) / len(rq)
+        kinds[kind] = {"kernel": k, "grid_threads": grid, "dispatches": len(c["FETCH_SIZE"]), "fetch_kib": f,
+                       "write_kib": w, "hbm_bytes_per_launch": (f * factor["FETCH_SIZE"] + w * factor["WRITE_SIZE"]) * 1024,
+                       "read_bytes_from_request_sizes": rbytes,
+                       "algorithmic_bytes_per_launch": line["passes"][kind]["GB_per_launch"] * 1e9}
+    with open(out_json, "w") as fh:
+        json.dump({"method": "rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), each scaled by "
+                             "what it reads for a 1 GiB hipMemcpy device-to-device copy",
+                   "command": command, "fetch_factor": factor["FETCH_SIZE"], "write_factor": factor["WRITE_SIZE"],
+                   "kinds": kinds}, fh, indent=1)
+    out = [f"Calibration (1 GiB device-to-device copies): FETCH_SIZE x {factor['FETCH_SIZE']:.3f}, "
+           f"WRITE_SIZE x {factor['WRITE_SIZE']:.3f}.", "",
+           "| pass | kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM GB (scaled) | read GB from request sizes | "
+           "algorithmic GB | ratio |", "|---|---|---|---|---|---|---|---|"]
+    for kind, v in kinds.items():
+        rb = f"{v['read_bytes_from_request_sizes'] / 1e9:.3f}" if v["read_bytes_from_request_sizes"] else "--"
+        out.append(f"| {kind} | `{v['kernel']}` | {v['fetch_kib']:.0f} | {v['write_kib']:.0f} | "
+                   f"{v['hbm_bytes_per_launch'] / 1e9:.3f} | {rb} | {v['algorithmic_bytes_per_launch'] / 1e9:.3f} | "
+                   f"{v['hbm_bytes_per_launch'] / v['algorithmic_bytes_per_launch']:.2f} |")
+    return "\n".join(out) + "\n"
+
+
 def host_cpu():
     """(model name, physical cores) of the host, from lscpu."""
     model, cores = "unknown", os.cpu_count() or 1
@@ -665,6 +737,11 @@ def main():
     ap.add_argument("--trace-command", default="rocprofv3 --kernel-trace --marker-trace --stats --output-format csv "
                                                "-d gpurun_out/prof_c2 -o c2 -- python3 -u bench.py",
                     help="with --trace-summary: the command that made the trace (quoted in the summary)")
+    ap.add_argument("--traffic-summary", nargs=3, default=None, metavar=("FETCH_CSV", "WRITE_CSV", "RDREQ_CSV"),
+                    help="summarize the counter_collection.csv files of three rocprofv3 --pmc runs of this script "
+                         "(FETCH_SIZE; WRITE_SIZE; the TCC_EA0_RDREQ request sizes, or '-') into "
+                         "profiles/traffic_c2.json and a table on stdout (no GPU work), then exit; --bench-json names "
+                         "one run's JSON line, --trace-command the command the runs shared")
     ap.add_argument("--serve-slots", type=int, default=2,
                     help="lanes of the request loop (2: a spectrum build or a request overlaps another request; "
                          "more lanes thrash the caches shared by their amplitude tables)")
@@ -680,6 +757,11 @@ def main():
     args = ap.parse_args()
     if args.trace_summary:
         sys.stdout.write(trace_summary(args.trace_summary, args.marker_csv, args.bench_json, args.trace_command))
+        return
+    if args.traffic_summary:
+        f, w, r = args.traffic_summary
+        sys.stdout.write(traffic_summary(f, w, None if r == "-" else r, args.bench_json, args.trace_command,
+                                         os.path.join(ROOT, "profiles", "traffic_c2.json")))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this engine has no multi-GPU path (asked for --gpus {args.gpus}). "
