@@ -7,8 +7,10 @@
 //   - pass 0 reads its operands straight from HBM (or generates them: Hermitian coefficient
 //     fill) into registers, the last pass writes straight from registers; LDS is used only for
 //     the exchange between consecutive radix passes;
-//   - inside a radix pass the twiddles are compile-time constants; between passes one table
-//     twiddle per element (table: FP64 values rounded to FP32);
+//   - pass 0 has compile-time twiddles only; a later pass applies its table twiddles in tangent
+//     form, as radix-2 stages of six fused multiply-adds per butterfly (strided passes; the
+//     table is built in LDS from FP64 values rounded to FP32), or multiplies by W_n table
+//     entries (C2R passes, axes longer than 2^11);
 //   - the x pass is a real-output (C2R) transform computed as an n/2-point complex FFT of
 //     Z[k] = E[k] + i O[k], E = X[k] + conj X[n/2-k], O = (X[k] - conj X[n/2-k]) W_n^k, with
 //     x[2m] = Re z[m], x[2m+1] = Im z[m].  The imaginary parts of X[0] and X[n/2] are dropped,
@@ -154,19 +156,32 @@ __device__ __forceinline__ f2 coef_t(float amps, uint32_t st, bool conj, bool se
     }
     return mk(re, conj ? -im : im);
 }
+// a * b mod 2^31-1 without the final reduction: the state s or s + (2^31-1), below 2^32 (a, b
+// are nonzero states, so the product is no multiple of 2^31-1)
+__device__ __forceinline__ uint32_t pm_mul_lazy(uint32_t a, uint32_t b) {
+    uint64_t p = (uint64_t)a * (uint64_t)b;
+    return (uint32_t)(p & 0x7fffffffu) + (uint32_t)(p >> 31);
+}
+
 // The same for a site that is not self-conjugate, with the conjugation carried by the sign of
 // the amplitude (negative: conjugate): the real part takes |amplitude|, the imaginary part the
-// signed one -- the values of coef_t(|amps|, st, amps < 0, false), without a select.
+// signed one -- the values of coef_t(|amps|, st, amps < 0, false), without a select.  `st` is
+// pm_mul_lazy's state (s or s + M): only fractions of st/M are used, and a whole revolution more
+// does not change a sine or cosine:
+//   igauss=0: the phase st/M in [0, 2) revolutions;
+//   igauss=1: u1 = frac(st/M) = s/M (absolute error < 2^-52 before its rounding to FP32, so the
+//             logarithm's error stays below 5e-7 even at s = 1) and u2 = frac(st a/M).
 template <int IG>
 __device__ __forceinline__ f2 coef_signed(float amps, uint32_t st) {
-    const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
     if constexpr (IG == 0) {
+        const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
         const float amp = amps * 1.2011224087864498f;
         const float u = (float)st * inv_m;
         return mk(fabsf(amp) * __builtin_amdgcn_cosf(u), amp * __builtin_amdgcn_sinf(u));
     } else {
-        const float u1 = (float)st * inv_m;
-        const float u2 = (float)__builtin_amdgcn_fract((double)st * (16807.0 / 2147483647.0));
+        const double d = (double)st;
+        const float u1 = (float)__builtin_amdgcn_fract(d * (1.0 / 2147483647.0));
+        const float u2 = (float)__builtin_amdgcn_fract(d * (16807.0 / 2147483647.0));
         const float a = amps * __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
         return mk(fabsf(a) * __builtin_amdgcn_cosf(u2), a * __builtin_amdgcn_sinf(u2));
     }
@@ -256,6 +271,92 @@ __device__ __forceinline__ void radix_pass(f2* v, int g, TW tw) {
     }
 }
 
+// ------------------------------------------------------------ twiddled passes in tangent form
+// A Stockham pass J > 0 of radix R = 2^LR is LR radix-2 Stockham stages done in registers: stage s
+// pairs registers j and j + R/2^(s+1) of a butterfly with the twiddle W_{2^(s+1) NS}^(k + m NS),
+// k = b mod NS, m = brev_s(j >> (LR - s)) (s-bit reversal); the sum stays in j, the difference goes
+// to j + R/2^(s+1), so register j ends holding output brev_LR(j).  A twiddle w = c (1 + i t),
+// t = tan(arg w), makes each stage butterfly six fused multiply-adds,
+//     b' = (b.x - t b.y, b.y + t b.x),  a + w b = a + c b',  a - w b = a - c b',
+// against the table twiddle's complex product and the DFT_R's own constant rotations (72 against 80
+// operations per radix-8 butterfly).  cos(arg w) is never exactly 0 in the table (the FP32 values
+// of cos(pi/2) and cos(3 pi/2) are ~1e-16), and a large t is harmless: c b' then equals the product
+// to the table's own rounding.
+//
+// The table, in LDS: pass J from entry tan_base(J) on, one row of R - 1 float2 (c, t) per k < NS,
+// entry 2^s - 1 + m of a row for stage s: a butterfly reads its row from one address with immediate
+// offsets.  The passes J > 0 hold N - R_0 entries in all, no more than the W_n table.
+template <int L, int RM>
+constexpr int tan_base(int J) {
+    int e = 0;
+    for (int j = 1; j < J; ++j) e += (1 << nslog(L, j, RM)) * ((1 << rlog(L, j, RM)) - 1);
+    return e;
+}
+template <int L, int RM>
+constexpr int tan_entries() { return tan_base<L, RM>(npass(L, RM)); }
+
+// entries of the tangent table from the W_n table (wn: W_n^m, m < n, FP32), by the block's threads
+template <int L, int RM, int NT>
+__device__ __forceinline__ void tan_table_build(f2* tt, const f2* __restrict__ wn, const int tid) {
+    [&]<int... J>(std::integer_sequence<int, J...>) {
+        (
+            [&] {
+                constexpr int Jc = J + 1, LR = rlog(L, Jc, RM), R = 1 << LR, NSL = nslog(L, Jc, RM);
+                constexpr int E = (1 << NSL) * (R - 1), B = tan_base<L, RM>(Jc);
+                for (int e = tid; e < E; e += NT) {
+                    const int k = e / (R - 1), idx = e - k * (R - 1);
+                    const int s = 31 - __builtin_clz(idx + 1), m = idx + 1 - (1 << s);
+                    // W_{2^(s+1) NS}^(k + m NS) = W_n^((k + m NS) n / (2^(s+1) NS))
+                    const f2 x = __ldg(wn + ((k + (m << NSL)) << (L - s - 1 - NSL)));
+                    tt[B + e] = mk(x.x, x.y / x.x);
+                }
+            }(),
+            ...);
+    }(std::make_integer_sequence<int, npass(L, RM) - 1>{});
+}
+
+// a + w b and a - w b for w = c (1 + i t)
+__device__ __forceinline__ void bfly_tan(f2& a, f2& b, const float c, const float t) {
+    const float bx = __builtin_fmaf(-t, b.y, b.x), by = __builtin_fmaf(t, b.x, b.y);
+    const f2 p = mk(__builtin_fmaf(c, bx, a.x), __builtin_fmaf(c, by, a.y));
+    b = mk(__builtin_fmaf(-c, bx, a.x), __builtin_fmaf(-c, by, a.y));
+    a = p;
+}
+
+// Stockham pass J > 0 on the thread's butterflies (as radix_pass), twiddles from the tangent table
+template <int L, int J, int G, int RM>
+__device__ __forceinline__ void radix_pass_tan(f2* v, int g, const f2* tt) {
+    constexpr int LR = rlog(L, J, RM), R = 1 << LR, NS = 1 << nslog(L, J, RM);
+    constexpr int NB = (1 << eptlog(L, RM)) / R;
+    static_assert(J > 0 && NS > 1, "pass 0 has no twiddles");
+    const f2* tj = tt + tan_base<L, RM>(J);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int k = (g + G * i) & (NS - 1);
+        const f2* row = tj + k * (R - 1);
+        f2 ent[R - 1];
+#pragma unroll
+        for (int h = 0; h < R - 1; ++h) ent[h] = row[h];
+        f2* u = v + i * R;
+#pragma unroll
+        for (int s = 0; s < LR; ++s) {
+            const int d = R >> (s + 1);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                if (j & d) continue;
+                const int m = s == 0 ? 0 : brev(j >> (LR - s), s);
+                const f2 e = ent[(1 << s) - 1 + m];
+                bfly_tan(u[j], u[j + d], e.x, e.y);
+            }
+        }
+        f2 t[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) t[r] = u[brev(r, LR)];
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = t[r];
+    }
+}
+
 // twiddle table W_n^m read from HBM (cached) or from an LDS copy
 struct TwGlobal {
     const f2* __restrict__ p;
@@ -294,10 +395,14 @@ struct StridedCfg {
     static constexpr bool PADPOS = PAD;
     static constexpr int ppos(int pos) { return PADPOS ? pos + (pos >> EL) : pos; }
     static constexpr size_t lds_elems() { return (size_t)(N + (PADPOS ? (N >> EL) : 0)) * P; }
-    // lengths up to 2^11: the W_n table (n entries) is copied to LDS behind the exchange buffer,
-    // so that no pass waits on vector-memory loads (which complete in order with the stores)
+    // lengths up to 2^11: the passes J > 0 take their twiddles from a tangent table built in LDS
+    // behind the exchange buffer (N - R_0 entries), so that no pass waits on vector-memory loads
+    // (which complete in order with the stores); longer axes multiply by W_n table entries read
+    // from memory
     static constexpr bool TWL = L <= 11;
-    static constexpr size_t lds_bytes() { return lds_elems() * sizeof(f2) + (TWL ? (size_t)N * sizeof(f2) : 0); }
+    static constexpr size_t lds_bytes() {
+        return lds_elems() * sizeof(f2) + (TWL ? (size_t)tan_entries<L, RM>() * sizeof(f2) : 0);
+    }
 };
 
 __device__ __forceinline__ float4 pack2(f2 a, f2 b) { return make_float4(a.x, a.y, b.x, b.y); }
@@ -348,12 +453,8 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
     const int q = tid % PH, g = tid / PH;
     if constexpr (C::TWL && M > 1) {
         // first read after the barrier of the first exchange
-        for (int m = tid; m < N; m += C::NT) twl[m] = __ldg(tw + m);
+        tan_table_build<L, RM, C::NT>(twl, tw, tid);
     }
-    auto twid = [&] {
-        if constexpr (C::TWL) return TwLds{twl};
-        else return TwGlobal{tw};
-    }();
     const int i0base = (tile % a.ntiles0) * P;
     // element offsets of the tile's first column in the input and output (column-tile strides
     // sC: P for x-fastest buffers, the size of a column tile's slab for column-tiled ones)
@@ -393,8 +494,13 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                                 v1[e] = mk(w.z, w.w);
                             }
                     }
-                    radix_pass<L, Jc, G, RM>(v0, g, twid);
-                    radix_pass<L, Jc, G, RM>(v1, g, twid);
+                    if constexpr (Jc > 0 && C::TWL) {
+                        radix_pass_tan<L, Jc, G, RM>(v0, g, twl);
+                        radix_pass_tan<L, Jc, G, RM>(v1, g, twl);
+                    } else {
+                        radix_pass<L, Jc, G, RM>(v0, g, TwGlobal{tw});
+                        radix_pass<L, Jc, G, RM>(v1, g, TwGlobal{tw});
+                    }
                 }(),
                 ...);
         }(std::make_integer_sequence<int, M>{});
@@ -552,8 +658,8 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                 // keep the site values opaque so that nothing derived from them is hoisted
                 // out of the member loop (registers)
                 asm volatile("" : "+v"(w0[r]), "+v"(w1[r]), "+v"(ampv[r].x), "+v"(ampv[r].y));
-                v0[r] = coef_signed<IG>(ampv[r].x, pm_mul(w0[r], rb));
-                v1[r] = coef_signed<IG>(ampv[r].y, pm_mul(w1[r], rb));
+                v0[r] = coef_signed<IG>(ampv[r].x, pm_mul_lazy(w0[r], rb));
+                v1[r] = coef_signed<IG>(ampv[r].y, pm_mul_lazy(w1[r], rb));
             }
             if (col0) {  // kx = 0: self-conjugate sites; the Nyquist column (kx = npx/2, its own partner)
                 int zero = 0;  // opaque: the addresses below are not kept across the loop
