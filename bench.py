@@ -752,6 +752,8 @@ def main():
     ap.add_argument("--plan", default="",
                     help="radix plans Z,Y,X (log2 of the largest radix of the FP32 z fill, y and C2R x passes) "
                          "instead of autotuning, so that profiling runs launch the same kernels")
+    ap.add_argument("--fuse", type=int, default=-1,
+                    help="SS3D_OPT_FUSE of the engine (0: separate launches, 1: fused); default: the library's")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
@@ -780,6 +782,8 @@ def main():
         hip.free(a)
         hip.free(b)
     eng = S.Engine(0)
+    if args.fuse >= 0:
+        eng.set_option(S.OPT_FUSE, args.fuse)
     if args.serve_only:
         # the radix plans the full run would have tuned for this grid
         spec = eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *S.periodic_grid(STRUCTS, 1.0, 1.0, 1.0, *GRID, 1))
@@ -843,23 +847,31 @@ def main():
     regions.pause()
     eng.profile(False)
     prof32 = eng.profile_read()
-    hip.free(out32)
 
-    # ---- FP64 drop-in path (same batch, FP64 output)
-    t64, prof64 = None, {}
+    # ---- FP64 drop-in path (same batch, FP64 output); its last timed step realizes the draws of
+    # the last FP32 step, so that realization 0 of both compares the FP32 field with the FP64 one
+    t64, prof64, rel32 = None, {}, None
     if args.fp64_steps > 0:
         out64 = hip.malloc(nreal * ncell * 8)
+        last = args.warmup + args.steps - 1
         run(S.FP64, out64, 0, nreal)
         eng.synchronize()
         eng.profile(True)
         t0 = time.perf_counter()
         for k in range(args.fp64_steps):
-            run(S.FP64, out64, 1 + k, nreal)
+            run(S.FP64, out64, max(0, last - args.fp64_steps + 1 + k), nreal)
         eng.synchronize()
         t64 = time.perf_counter() - t0
         eng.profile(False)
         prof64 = eng.profile_read()
+        import numpy as np
+
+        f64, f32 = np.empty(ncell, np.float64), np.empty(ncell, np.float32)
+        hip.copy_d2h(f64.ctypes.data, out64, ncell * 8)
+        hip.copy_d2h(f32.ctypes.data, out32, ncell * 4)
+        rel32 = float(np.sqrt(np.mean((f32.astype(np.float64) - f64) ** 2) / np.mean(f64 ** 2)))
         hip.free(out64)
+    hip.free(out32)
     hip.free(stats)
     writer = None if args.no_writer else writer_leg(S, eng, spec, nreal)
     specfile = None if args.no_writer else spectrum_file_leg(S, eng, spec, grid, spec_ms)
@@ -905,6 +917,7 @@ def main():
         "fp64_plain": ({"value": round(fp64_value, 3), "unit": "Gcells/s", "steps": args.fp64_steps,
                         "ms_per_realization": round(t64 / (args.fp64_steps * nreal) * 1e3, 4)} if t64 else None),
         "fp32_over_fp64": round(value / fp64_value, 2) if t64 else None,
+        "fp32_vs_fp64_rel_rms": rel32,
         "spectrum_ms": round(spec_ms, 2),
         "autotune": {"largest_radix_log2": plan, "plans_and_tiles": plans_tiles,
                      "untuned_gcells_s": round(untuned, 3) if untuned else None, "tuned_gcells_s": round(value, 3),
@@ -921,6 +934,14 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kind),
                      "algorithmic_bytes_per_launch": byt / nl, "ms_per_launch": ms / nl},
         "passes": passes,
+    }
+    # the stated targets, against this run
+    frac = achieved / HBM_PEAK_GBS
+    line["targets"] = {
+        "fp32_over_fp64_plain": {"target": ">= 10", "value": line["fp32_over_fp64"],
+                                 "met": bool(t64) and value / fp64_value >= 10},
+        "dominant_pass_hbm_frac": {"target": ">= 0.60", "value": round(frac, 4), "met": frac >= 0.60},
+        "fp32_rel_rms_vs_fp64": {"target": "<= 1e-5", "value": rel32, "met": rel32 is not None and rel32 <= 1e-5},
     }
     if not args.no_cpu:
         cb = cpu_baseline()
