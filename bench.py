@@ -754,6 +754,8 @@ def main():
                          "instead of autotuning, so that profiling runs launch the same kernels")
     ap.add_argument("--fuse", type=int, default=-1,
                     help="SS3D_OPT_FUSE of the engine (0: separate launches, 1: fused); default: the library's")
+    ap.add_argument("--zrun", type=int, default=-1,
+                    help="SS3D_OPT_ZRUN of the engine (0: row layouts, 1: z-run layout); default: the library's")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run device-to-device copies of this many MiB (hardware-counter calibration)")
     args = ap.parse_args()
@@ -784,6 +786,8 @@ def main():
     eng = S.Engine(0)
     if args.fuse >= 0:
         eng.set_option(S.OPT_FUSE, args.fuse)
+    if args.zrun >= 0:
+        eng.set_option(S.OPT_ZRUN, args.zrun)
     if args.serve_only:
         # the radix plans the full run would have tuned for this grid
         spec = eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *S.periodic_grid(STRUCTS, 1.0, 1.0, 1.0, *GRID, 1))

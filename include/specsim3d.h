@@ -190,6 +190,12 @@ int ss3d_tuning_set_tile(ss3d_context* ctx, int kind, int logn, int rm, int vari
 /* SS3D_OPT_BATCH_AUTOTUNE (default 1): ss3d_batch_create autotunes every distinct grid of its
  * jobs; 0 keeps the context's tuning table as it is (for comparisons with untuned plans). */
 #define SS3D_OPT_BATCH_AUTOTUNE 4
+/* SS3D_OPT_ZRUN (default 1): FP32 realizations whose z and y axes have length 2^9 keep the fill
+ * and y outputs in runs of 8 z per pair of kx columns (cropped nz a multiple of 8), written by a
+ * z fill whose waves exchange without block barriers and read by a C2R pass that takes whole runs;
+ * 0 keeps the row layouts.  Same field values bit for bit (the statistics' block sums group other
+ * rows, so they may differ in the last bits). */
+#define SS3D_OPT_ZRUN 5
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
 /* ---------------------------------------------------------------- request loop

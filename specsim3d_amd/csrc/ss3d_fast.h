@@ -441,20 +441,43 @@ __device__ __forceinline__ void st_nt16(float4 v, float4* p) {
 // HALF (passes that keep at most half the axis, n_out <= N/2): the last radix pass computes
 // only the outputs r < R/2 of each butterfly.  Output r of a last-pass butterfly sits at position
 // pos0 + r N/R with pos0 < N/R, so the outputs r >= R/2 lie at N/2 or beyond and are never stored.
-template <int L, int P, int IN, int AXIS, int RM, int IG, bool HALF = false, bool PAD = (P < 16)>
+//
+// ZR (z fill only): wave-local z-run fill.  Thread tid holds column pair q = tid / G and butterfly
+// slot g = tid % G, so a wave holds whole column pairs: both LDS exchanges of a member stay inside
+// the wave, in the pairs' own padded slots of the exchange buffer, ordered by wave-scope fences
+// (the LDS executes a wave's instructions in order), and the member loop has no block barrier.
+// The output goes to the z-run layout: element z of column pair p in row `third` at float4
+// (z / 8) sZ + p sP + third sK + z mod 8 (sZ, sP, sK = out_sA, out_sC, out_sT in float4 units), so
+// that the 64 lanes of a store write whole runs of 8 z (128 bytes).  The site values are computed
+// in the standard mapping (coalesced amplitude and row-factor reads) and moved to the wave-local
+// one through LDS once per block.  Every column's arithmetic is that of the standard mapping.
+template <int L, int P, int IN, int AXIS, int RM, int IG, bool HALF = false, bool PAD = (P < 16), bool ZR = false>
 __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, const int grp) {
-    using C = StridedCfg<L, P, RM, PAD>;
+    using C = StridedCfg<L, P, RM, PAD || ZR>;
     constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, PH = P / 2;
     static_assert(M > 0, "strided passes have length >= 2");
+    static_assert(!ZR || (IN == IN_FILL32H && AXIS == 2 && 64 % G == 0 && C::TWL), "z-run: wave-local z fill");
+    // slot of one column pair in the exchange buffer (z-run: padded positions, pair-major)
+    constexpr int ZSLOT = N + (N >> C::EL);
+    static_assert(!ZR || (size_t)ZSLOT * PH * 2 == C::lds_elems(), "z-run slots fill the exchange buffer");
     float4* lds4 = reinterpret_cast<float4*>(fast_smem);
     const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);
     f2* twl = reinterpret_cast<f2*>(fast_smem + C::lds_elems() * sizeof(f2));
     const int tid = threadIdx.x;
-    const int q = tid % PH, g = tid / PH;
+    const int q = ZR ? tid / G : tid % PH, g = ZR ? tid % G : tid / PH;
     if constexpr (C::TWL && M > 1) {
-        // first read after the barrier of the first exchange
+        // first read after the barrier of the first exchange (z-run: after the site transposition)
         tan_table_build<L, RM, C::NT>(twl, tw, tid);
     }
+    // exchange slot of position pos of pair qq, and the order between an exchange's writes and reads
+    auto xslot = [](const int pos, const int qq) __attribute__((always_inline)) {
+        if constexpr (ZR) return qq * ZSLOT + C::ppos(pos);
+        else return C::ppos(pos) * PH + qq;
+    };
+    auto xsync = [] __attribute__((always_inline)) {
+        if constexpr (ZR) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        else __syncthreads();
+    };
     const int i0base = (tile % a.ntiles0) * P;
     // element offsets of the tile's first column in the input and output (column-tile strides
     // sC: P for x-fastest buffers, the size of a column tile's slab for column-tiled ones)
@@ -475,21 +498,21 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                     if constexpr (Jc > 0) {
                         constexpr int LRp = rlog(L, Jc - 1, RM), NBp = EPT >> LRp;
                         constexpr int LR = rlog(L, Jc, RM), NB = EPT >> LR;
-                        if constexpr (Jc > 1) __syncthreads();
+                        if constexpr (Jc > 1) xsync();
 #pragma unroll
                         for (int i = 0; i < NBp; ++i)
 #pragma unroll
                             for (int r = 0; r < (1 << LRp); ++r) {
                                 const int e = i * (1 << LRp) + r;
-                                lds4[C::ppos(out_pos<L, Jc - 1, RM>(g + G * i, r)) * PH + q] = pack2(v0[e], v1[e]);
+                                lds4[xslot(out_pos<L, Jc - 1, RM>(g + G * i, r), q)] = pack2(v0[e], v1[e]);
                             }
-                        __syncthreads();
+                        xsync();
 #pragma unroll
                         for (int i = 0; i < NB; ++i)
 #pragma unroll
                             for (int r = 0; r < (1 << LR); ++r) {
                                 const int e = i * (1 << LR) + r;
-                                const float4 w = lds4[C::ppos(g + G * i + r * (N >> LR)) * PH + q];
+                                const float4 w = lds4[xslot(g + G * i + r * (N >> LR), q)];
                                 v0[e] = mk(w.x, w.y);
                                 v1[e] = mk(w.z, w.w);
                             }
@@ -519,6 +542,28 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                     const uint32_t off = (active && pos < nout) ? (q + (uint32_t)pos * sA) * 16u : kOob;
                     if (!HALF || r < (1 << LRl) / 2) bstore16(pack2(v0[e], v1[e]), rs, off);
                 }
+        } else if constexpr (ZR) {
+            // z-run stores through a resource spanning the member's buffer; pruned positions and
+            // inactive lanes use an out-of-range offset, and a store whose positions are pruned for
+            // the whole wave is not issued
+            constexpr int NSLl = nslog(L, M - 1, RM);
+            const uint32_t sZ = (uint32_t)(a.out_sA >> 1), sP = (uint32_t)(a.out_sC >> 1), sK = (uint32_t)(a.out_sT >> 1);
+            const __amdgpu_buffer_rsrc_t rs =
+                rsrc(reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride, (int)(a.out_bstride * 8));
+            const uint32_t o0 = (uint32_t)(i0base / 2 + q) * sP + (uint32_t)third * sK;
+#pragma unroll
+            for (int i = 0; i < NBl; ++i) {
+                const int pos0 = out_pos<L, M - 1, RM>(g + G * i, 0);
+                const int wpos0 = __builtin_amdgcn_readfirstlane(pos0);
+#pragma unroll
+                for (int r = 0; r < (1 << LRl); ++r) {
+                    const int e = i * (1 << LRl) + r;
+                    const int z = pos0 + (r << NSLl);
+                    const uint32_t off =
+                        active && z < nout ? (o0 + (uint32_t)(z >> 3) * sZ + (uint32_t)(z & 7)) * 16u : kOob;
+                    if ((!HALF || r < (1 << LRl) / 2) && wpos0 + (r << NSLl) < nout) bstore16(pack2(v0[e], v1[e]), rs, off);
+                }
+            }
         } else {
             // fill passes: element r of butterfly i sits at position pos(i, 0) + r 2^NSL.  When the
             // axis spans less than 2 GiB the stores go through a buffer resource that ends at
@@ -610,45 +655,94 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
         uint32_t selfm = 0, conjm = 0;
         const bool col0 = active && c0 == 0, pack = col0 && a.npx >= 2;
         const int kxn = a.npx >> 1;
-        uint32_t txn = 0;
-        // amplitude of element r: amp32h[c0 + hxa * row(pos)], row affine in r
-        const int64_t arow0 = AXIS == 2 ? (int64_t)third + (int64_t)a.npy * g : (int64_t)g + (int64_t)a.npy * third;
+        const uint32_t txn = pack ? __ldg(a.txp + kxn) : 0u;
+        // amplitude of element r of the pair at column cc, slot gg: amp32h[cc + hxa * row(pos)], row
+        // affine in r
+        auto amp_base = [&](const int cc, const int gg) __attribute__((always_inline)) {
+            const int64_t arow0 =
+                AXIS == 2 ? (int64_t)third + (int64_t)a.npy * gg : (int64_t)gg + (int64_t)a.npy * third;
+            return a.amp32h + cc + a.hxa * arow0;
+        };
         const int64_t arstep = AXIS == 2 ? (int64_t)a.npy * G : G;
-        const float* __restrict__ abase = a.amp32h + c0 + a.hxa * arow0;
+        const float* __restrict__ abase = amp_base(c0, g);
         const int64_t astep = a.hxa * arstep;
-        if (active) {
-            const int kxa = c0, kxb = c0 + 1;
-            const int pa = (a.npx - kxa) & (a.npx - 1), pb = (a.npx - kxb) & (a.npx - 1);
-            const uint32_t txa = __ldg(a.txp + kxa), txpa = __ldg(a.txp + pa);
-            const uint32_t txb = __ldg(a.txp + kxb), txpb = __ldg(a.txp + pb);
-            if (pack) txn = __ldg(a.txp + kxn);
+        // the site values of column pair qq at slot gg
+        auto site_values = [&](const int qq, const int gg) __attribute__((always_inline)) {
+            const int cc = i0base + 2 * qq;
+            selfm = conjm = 0;
+            if (cc < a.n0) {
+                const float* __restrict__ ab = amp_base(cc, gg);
+                const int kxa = cc, kxb = cc + 1;
+                const int pa = (a.npx - kxa) & (a.npx - 1), pb = (a.npx - kxb) & (a.npx - 1);
+                const uint32_t txa = __ldg(a.txp + kxa), txpa = __ldg(a.txp + pa);
+                const uint32_t txb = __ldg(a.txp + kxb), txpb = __ldg(a.txp + pb);
 #pragma unroll
-            for (int r = 0; r < EPT; ++r) {
-                const int pos = g + r * G;
-                const Site sv = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
-                const uint32_t rf = __ldg(a.rowf + sv.site);
-                const float2 am = __ldg(reinterpret_cast<const float2*>(abase + r * astep));
-                // the site row is shared by every column; x order decides only inside self rows
-                const bool ja = sv.gt || (sv.eq && kxa > pa);
-                const bool jb = sv.gt || (sv.eq && kxb > pb);
-                w0[r] = pm_mul(rf, ja ? txpa : txa);
-                w1[r] = pm_mul(rf, jb ? txpb : txb);
-                ampv[r] = make_float2(ja ? -am.x : am.x, jb ? -am.y : am.y);
-                if (sv.eq) selfm |= 1u << r;
-                if (ja) conjm |= 1u << r;
+                for (int r = 0; r < EPT; ++r) {
+                    const int pos = gg + r * G;
+                    const Site sv = AXIS == 2 ? site_of(a, third, pos) : site_of(a, pos, third);
+                    const uint32_t rf = __ldg(a.rowf + sv.site);
+                    const float2 am = __ldg(reinterpret_cast<const float2*>(ab + r * astep));
+                    // the site row is shared by every column; x order decides only inside self rows
+                    const bool ja = sv.gt || (sv.eq && kxa > pa);
+                    const bool jb = sv.gt || (sv.eq && kxb > pb);
+                    w0[r] = pm_mul(rf, ja ? txpa : txa);
+                    w1[r] = pm_mul(rf, jb ? txpb : txb);
+                    ampv[r] = make_float2(ja ? -am.x : am.x, jb ? -am.y : am.y);
+                    if (sv.eq) selfm |= 1u << r;
+                    if (ja) conjm |= 1u << r;
+                }
+            } else {
+                // inactive lanes: amplitude 0 and a valid state (coefficient 0), so that the member
+                // loop runs without a branch on the lane's column
+#pragma unroll
+                for (int r = 0; r < EPT; ++r) {
+                    w0[r] = w1[r] = 1u;
+                    ampv[r] = make_float2(0.f, 0.f);
+                }
             }
+        };
+        if constexpr (ZR) {
+            // computed in the standard mapping (pair tid % PH, slot tid / PH), then moved to the
+            // wave-local one through each pair's exchange slot: G * EPT values of 16 bytes, then
+            // the two masks of each slot.  Value r of slot gg of pair qq sits at gg EPT +
+            // (r + qq + gg / 2) mod EPT: the 16 lanes of a 16-byte LDS access (8 pairs x 2 slots
+            // when written, 16 slots of one pair when read) then fall in distinct banks.
+            uint4* xs = reinterpret_cast<uint4*>(fast_smem);
+            static_assert(G * EPT * 16 + G * 8 <= ZSLOT * 16, "site values inside a pair's slot");
+            auto vslot = [](const int qq, const int gg, const int r) __attribute__((always_inline)) {
+                return qq * ZSLOT + gg * EPT + ((r + qq + (gg >> 1)) & (EPT - 1));
+            };
+            {
+                const int qs = tid % PH, gs = tid / PH;
+                site_values(qs, gs);
+#pragma unroll
+                for (int r = 0; r < EPT; ++r)
+                    xs[vslot(qs, gs, r)] = make_uint4(w0[r], w1[r], __float_as_uint(ampv[r].x), __float_as_uint(ampv[r].y));
+                uint32_t* dm = reinterpret_cast<uint32_t*>(xs + qs * ZSLOT + G * EPT) + 2 * gs;
+                dm[0] = selfm;
+                dm[1] = conjm;
+            }
+            __syncthreads();  // also orders the tangent table's writes before its first read
+            {
+#pragma unroll
+                for (int r = 0; r < EPT; ++r) {
+                    const uint4 u = xs[vslot(q, g, r)];
+                    w0[r] = u.x;
+                    w1[r] = u.y;
+                    ampv[r] = make_float2(__uint_as_float(u.z), __uint_as_float(u.w));
+                }
+                const uint32_t* sm = reinterpret_cast<const uint32_t*>(xs + q * ZSLOT + G * EPT) + 2 * g;
+                selfm = sm[0];
+                conjm = sm[1];
+            }
+            // no block barrier before the first exchange: a wave writes only its own pairs' slots,
+            // whose moved values only that wave reads, in order
         } else {
-            // inactive lanes: amplitude 0 and a valid state (coefficient 0), so that the member
-            // loop runs without a branch on the lane's column
-#pragma unroll
-            for (int r = 0; r < EPT; ++r) {
-                w0[r] = w1[r] = 1u;
-                ampv[r] = make_float2(0.f, 0.f);
-            }
+            site_values(q, g);
         }
         // the member loop, straight-line for the coefficient law IG (igauss) of the instantiation
         for (int b = b0; b < b1; ++b) {
-            if (b > b0) __syncthreads();  // LDS reads of the previous member are done
+            if (b > b0) xsync();  // LDS reads of the previous member are done
             // realization base: a uniform read through the scalar cache (written by an earlier
             // kernel, never by this one), so that no vector-memory wait (which would also wait
             // for the stores of the previous member) is needed
@@ -685,9 +779,9 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
     }
 }
 
-template <int L, int P, int IN, int AXIS, int RM, int IG, bool PAD = (P < 16)>
+template <int L, int P, int IN, int AXIS, int RM, int IG, bool PAD = (P < 16), bool ZR = false>
 __global__ __launch_bounds__((StridedCfg<L, P, RM, PAD>::NT)) void strided_kernel(PassArgs a) {
-    strided_tile<L, P, IN, AXIS, RM, IG, false, PAD>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.y);
+    strided_tile<L, P, IN, AXIS, RM, IG, false, PAD, ZR>(a, xcd_remap(blockIdx.x, gridDim.x), blockIdx.y);
 }
 
 // ------------------------------------------------------------ fill pass beside a y pass
@@ -696,8 +790,9 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM, PAD>::NT)) void strided_kerne
 // interleave -- 8A fill blocks, then 8C y blocks, K times, then the rest of each -- so that every
 // CU holds arithmetic-bound fill tiles and memory-bound y tiles at once.  Role-local block l runs
 // tile l mod nblocks (in XCD order) of member group l / nblocks of its pass; 8A and 8C keep l and
-// the block index equal mod 8 (the same XCD).
-template <int L, int P, int RM, int IG, int LY = L, int PY = P, int RMY = RM, bool HALF = false>
+// the block index equal mod 8 (the same XCD).  ZR: the fill role is the wave-local z-run fill
+// (the y role then reads the z-run layout through its strides, as any other layout).
+template <int L, int P, int RM, int IG, int LY = L, int PY = P, int RMY = RM, bool HALF = false, bool ZR = false>
 __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(PassArgs az, PassArgs ay, FuseMap m) {
     static_assert(StridedCfg<LY, PY, RMY>::NT == StridedCfg<L, P, RM>::NT, "tiles of one thread count");
     const int per = 8 * (m.A + m.C), bid = blockIdx.x, full = m.K * per;
@@ -713,7 +808,8 @@ __global__ __launch_bounds__((StridedCfg<L, P, RM>::NT)) void fill_load_kernel(P
         l = fill ? 8 * m.A * m.K + o : 8 * m.C * m.K + (o - zr);
     }
     if (fill)
-        strided_tile<L, P, IN_FILL32H, 2, RM, IG, HALF>(az, xcd_remap(l % az.nblocks, az.nblocks), l / az.nblocks);
+        strided_tile<L, P, IN_FILL32H, 2, RM, IG, HALF, (P < 16), ZR>(az, xcd_remap(l % az.nblocks, az.nblocks),
+                                                                     l / az.nblocks);
     else
         strided_tile<LY, PY, IN_LOAD, 1, RMY, 0, HALF>(ay, xcd_remap(l % ay.nblocks, ay.nblocks), l / ay.nblocks);
 }
@@ -1051,6 +1147,152 @@ template <int L, int RM, bool EVEN>
 __global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_loop_kernel(PassArgs a) {
     const int b0 = blockIdx.y * a.bpb;
     c2r_rows<L, RM, EVEN>(a, blockIdx.x, b0, min(b0 + a.bpb, a.nbatch));
+}
+
+// The C2R pass from a z-run y output: X[kx] of row (y, z) is component kx mod 2 of float4
+// (z / 8) sZ + y sY + (kx / 2) sP + z mod 8, with sZ, sY, sP = in_sT, in_sA, in_sC in float4
+// units.  A block's tile is P = 8 YB rows: the 8 z of one run at YB consecutive y.  Its loads
+// read whole 128-byte runs (lanes along z mod 8, then the pair, then y) and put each pair into its
+// row in LDS, from where thread (q, g) reads X[k] and X[N-k], k = g + r G, of row q -- the values
+// c2r_mirror takes from the lanes of a row.  Operands, radix passes, stores and statistics are
+// those of c2r_rows (the values of a row do not depend on its tile); the block sums have their own
+// LDS slots, so a member waits at four block barriers: after the rows' writes, after their reads
+// (before the first exchange writes), in the exchange, and for the block sums.
+template <int L, int RM>
+struct C2RZrCfg {
+    using C = C2RCfg<L, RM>;
+    static constexpr int N = C::N, P = C::P, HP = N / 2, YB = P / 8, NW = (C::NT + 63) / 64;
+    static constexpr int NL = P * HP / C::NT;  // 16-byte loads per thread and member
+    // row stride of the moved rows, complex: even (16-byte pair writes) and 2 RW = 40 mod 64
+    // words, so that the 8 rows of a run written by one instruction fall in distinct banks
+    static constexpr int RW = N + 20;
+    static constexpr size_t xbuf_elems() {
+        return C::lds_elems() > (size_t)P * RW ? C::lds_elems() : (size_t)P * RW;
+    }
+    static constexpr size_t lds_bytes() {
+        return (xbuf_elems() + 2 * (size_t)N) * sizeof(f2) + 2 * (size_t)NW * sizeof(double);
+    }
+    static constexpr bool ok = C::G <= 64 && C::M > 0 && P % 8 == 0 && (P * HP) % C::NT == 0;
+};
+template <int L, int RM, bool EVEN>
+__device__ __forceinline__ void c2r_rows_zr(const PassArgs& a, const int rblock, const int b0, const int b1) {
+    using C = C2RCfg<L, RM>;
+    using Z = C2RZrCfg<L, RM>;
+    constexpr int N = C::N, EPT = C::EPT, G = C::G, M = C::M, NT = C::NT;
+    constexpr int HP = Z::HP, YB = Z::YB, NL = Z::NL, RW = Z::RW, NW = Z::NW;
+    static_assert(Z::ok, "rows inside one wave, whole runs per tile");
+    f2* lds = reinterpret_cast<f2*>(fast_smem);
+    f2* twl = lds + Z::xbuf_elems();  // W_N (N entries), then the first N entries of W_{2N}
+    double* red = reinterpret_cast<double*>(twl + 2 * N);
+    const int tid = threadIdx.x;
+    const int g = tid % G, q = tid / G;  // row q of the tile: y0 + q / 8, z = 8 zb + q mod 8
+    const int nyb = a.ny / YB, zb = rblock / nyb, y0 = (rblock - zb * nyb) * YB;
+    {
+        const f2* __restrict__ tw = reinterpret_cast<const f2*>(a.tw);
+        for (int m = tid; m < 2 * N; m += NT) twl[m] = __ldg(tw + m);
+    }
+    const uint32_t sY = (uint32_t)(a.in_sA >> 1), sP = (uint32_t)(a.in_sC >> 1);
+    const int64_t tile0 = 2 * ((int64_t)zb * (a.in_sT >> 1) + (int64_t)y0 * sY);  // complex
+    // load j of thread tid: float4 f = j NT + tid of the tile, run position f mod 8, pair
+    // (f / 8) mod HP, row y0 + f / (8 HP); the pair goes to LDS row 8 (f / (8 HP)) + f mod 8
+    auto src_off = [&](const int j) __attribute__((always_inline)) {
+        const int f = j * NT + tid;
+        return (uint32_t)((f >> 3) / HP) * sY + (uint32_t)((f >> 3) & (HP - 1)) * sP + (uint32_t)(f & 7);
+    };
+    auto lds_pos = [&](const int j) __attribute__((always_inline)) {
+        const int f = j * NT + tid;
+        return (((f >> 3) / HP) * 8 + (f & 7)) * RW + 2 * ((f >> 3) & (HP - 1));
+    };
+    float4 nxt[NL];
+    auto load = [&](int b) {  // b >= b1: zeros, no memory access
+        const int bb = min(b, b1 - 1);
+        const __amdgpu_buffer_rsrc_t rs =
+            rsrc(reinterpret_cast<const f2*>(a.in) + (int64_t)bb * a.in_bstride + tile0);
+        const bool ok = b < b1;
+#pragma unroll
+        for (int j = 0; j < NL; ++j) nxt[j] = bload16(rs, ok ? src_off(j) * 16u : kOob);
+    };
+    constexpr int LRl = rlog(L, M - 1, RM), NBl = EPT >> LRl;
+    load(b0);
+    {
+        // as many out-of-range (dropped) stores as a member issues (see c2r_rows)
+        const __amdgpu_buffer_rsrc_t rs = rsrc(a.out);
+#pragma unroll
+        for (int e = 0; e < EPT + 3; ++e) bstore8(mk(0.f, 0.f), rs, kOob + 8u * e);
+        if constexpr (!EVEN) {
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) bstore4(0.f, rs, kOob + 4u * e);
+        }
+    }
+    const uint32_t obase = (uint32_t)((8 * zb + (q & 7)) * a.ny + y0 + (q >> 3)) * (uint32_t)a.out_row;
+    const int nx = a.n_out;
+    const int lane = tid & 63, wid = tid >> 6;
+    for (int b = b0; b < b1; ++b) {
+        f2 xa[EPT], xb[EPT], v[EPT];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) *reinterpret_cast<float4*>(lds + lds_pos(j)) = nxt[j];
+        load(b + 1);  // in flight while member b is transformed and stored
+        __syncthreads();  // the rows are in LDS (and, before the first member, the twiddles)
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) xa[r] = lds[q * RW + g + r * G];
+#pragma unroll
+        for (int r = 0; r < EPT; ++r) {
+            const int k = g + r * G;
+            // X[N] rides in X[0].y (packed Nyquist column)
+            xb[r] = k == 0 ? mk(xa[r].y, 0.f) : lds[q * RW + (N - k)];
+        }
+        if (g == 0) xa[0].y = 0.f;
+        __syncthreads();  // the rows are read before the exchange buffer is written
+        c2r_operands<EPT, G>(xa, xb, TwLds{twl + N}, g, v);
+        c2r_passes<L, RM>(v, lds, q, g, TwLds{twl});
+        double s1 = 0.0, s2 = 0.0;
+        const __amdgpu_buffer_rsrc_t ro = rsrc(reinterpret_cast<float*>(a.out) + (int64_t)b * a.out_bstride);
+#pragma unroll
+        for (int i = 0; i < NBl; ++i)
+#pragma unroll
+            for (int r = 0; r < (1 << LRl); ++r) {
+                const int m = out_pos<L, M - 1, RM>(g + G * i, r);
+                const f2 z = v[i * (1 << LRl) + r];
+                const bool ok = 2 * m + 1 < nx;
+                if constexpr (EVEN) {
+                    bstore8(z, ro, ok ? (obase + 2 * m) * 4u : kOob);
+                } else {
+                    const bool okx = 2 * m < nx;
+                    bstore4(z.x, ro, okx ? (obase + 2 * m) * 4u : kOob);
+                    bstore4(z.y, ro, ok ? (obase + 2 * m + 1) * 4u : kOob);
+                }
+                if (ok) {
+                    s1 += (double)z.x + (double)z.y;
+                    s2 += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+                } else if (!EVEN && 2 * m < nx) {
+                    s1 += (double)z.x;
+                    s2 += (double)z.x * (double)z.x;
+                }
+            }
+        s1 = wsum(s1);
+        s2 = wsum(s2);
+        if (lane == 0) {
+            red[wid] = s1;
+            red[NW + wid] = s2;
+        }
+        __syncthreads();  // the block sums (also: every exchange read is done before the next rows)
+        double t1 = 0, t2 = 0;
+        for (int w = 0; w < NW; ++w) {
+            t1 += red[w];
+            t2 += red[NW + w];
+        }
+        const __amdgpu_buffer_rsrc_t rp = rsrc(a.partial + 3 * ((int64_t)b * a.nblocks + rblock));
+        const uint32_t po = tid == 0 ? 0u : kOob;
+        bstore8d(t1, rp, po);
+        bstore8d(t2, rp, po + 8u);
+        bstore8d(0.0, rp, po + 16u);
+    }
+}
+
+template <int L, int RM, bool EVEN>
+__global__ __launch_bounds__((C2RCfg<L, RM>::NT)) void c2r_zr_kernel(PassArgs a) {
+    const int b0 = blockIdx.y * a.bpb;
+    c2r_rows_zr<L, RM, EVEN>(a, blockIdx.x, b0, min(b0 + a.bpb, a.nbatch));
 }
 
 // 1-point x axis (npx = 1): real part of the only coefficient of each row.

@@ -106,10 +106,65 @@ bool fusable(int L, int LY, int rmy, std::integer_sequence<int, Ls...>) {
     return ok;
 }
 
+// ---- z-run layout (PassArgs::zrun): the wave-local z fill of length 2^kZrunLog (tile 16), alone or
+// beside a y pass of the same length and plan, and the C2R pass reading the y pass's z-run output
+// (lengths 2^kZrunC2rMin .. 2^kZrunC2rMax, plans whose tiles hold whole runs)
+constexpr int kZrunLog = 9, kZrunC2rMin = 7, kZrunC2rMax = 10;
+
+template <typename K>
+hipError_t lds_attr(K k, size_t lds, bool& attr) {
+    if (lds > 65536 && !attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    return hipSuccess;
+}
+
+template <int L, int RM, int IG>
+hipError_t strided_zr_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    constexpr int P = strided_P(L);
+    using C = fast::StridedCfg<L, P, RM, true>;
+    static bool attr = false;
+    return launch_k(fast::strided_kernel<L, P, IN_FILL32H, 2, RM, IG, (P < 16), true>, C::lds_bytes(), C::NT, nblocks,
+                    nbatch, a, s, attr);
+}
+template <int RM>
+hipError_t strided_zr(int L, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    if (L != kZrunLog) return hipErrorInvalidValue;
+    return a.igauss ? strided_zr_L<kZrunLog, RM, 1>(nblocks, nbatch, a, s)
+                    : strided_zr_L<kZrunLog, RM, 0>(nblocks, nbatch, a, s);
+}
+
+template <int L, int RM, int IG, bool HALF>
+hipError_t fused_zr_L(int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    constexpr int P = strided_P(L);
+    using C = fast::StridedCfg<L, P, RM, true>;
+    using CY = fast::StridedCfg<L, P, RM>;
+    constexpr size_t lds = C::lds_bytes() > CY::lds_bytes() ? C::lds_bytes() : CY::lds_bytes();
+    static_assert(2 * lds <= 163840, "two blocks per CU");
+    static bool attr = false;
+    const auto k = fast::fill_load_kernel<L, P, RM, IG, L, P, RM, HALF, true>;
+    hipError_t e = lds_attr(k, lds, attr);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(C::NT), lds, s, az, ay, m);
+    return hipGetLastError();
+}
+template <int RM>
+hipError_t fused_zr(int L, int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
+    if (L != kZrunLog || LY != kZrunLog) return hipErrorInvalidValue;
+    constexpr int Lz = kZrunLog;
+    const bool half = 2 * az.n_out <= (1 << Lz) && 2 * (int64_t)ay.n_out <= (int64_t(1) << Lz);
+    if (az.igauss)
+        return half ? fused_zr_L<Lz, RM, 1, true>(nblocks, az, ay, m, s) : fused_zr_L<Lz, RM, 1, false>(nblocks, az, ay, m, s);
+    return half ? fused_zr_L<Lz, RM, 0, true>(nblocks, az, ay, m, s) : fused_zr_L<Lz, RM, 0, false>(nblocks, az, ay, m, s);
+}
+
 // z fill (az, length 2^L, plan RM) beside the y pass (ay, length 2^LY) in nblocks = m.nzb + m.nyb blocks
 template <int RM>
 hipError_t fused_rm(int L, int LY, int nblocks, const PassArgs& az, const PassArgs& ay, const FuseMap& m, hipStream_t s) {
     static_assert(2 * sizeof(PassArgs) + sizeof(FuseMap) <= 4096, "kernel arguments");
+    if (az.zrun) return fused_zr<RM>(L, LY, nblocks, az, ay, m, s);
     using seq = std::make_integer_sequence<int, kMaxFastLog>;
     return az.igauss ? fused_dispatch<RM, 1>(L, LY, nblocks, az, ay, m, s, seq{})
                      : fused_dispatch<RM, 0>(L, LY, nblocks, az, ay, m, s, seq{});
@@ -142,6 +197,40 @@ hipError_t c2r_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     return launch_k(fast::c2r_kernel<L, IN, RM>, C::lds_elems() * sizeof(float2), C::NT, nblocks, nbatch, a, s, attr);
 }
 
+// the C2R pass reading the z-run layout
+template <int L, int RM>
+hipError_t c2r_zr_L(int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
+    using Z = fast::C2RZrCfg<L, RM>;
+    if constexpr (Z::ok) {
+        static bool attr_even = false, attr_odd = false;
+        const int groups = (nbatch + a.bpb - 1) / a.bpb;
+        if (c2r_rows_even(a))
+            return launch_k(fast::c2r_zr_kernel<L, RM, true>, Z::lds_bytes(), Z::C::NT, nblocks, groups, a, s, attr_even);
+        return launch_k(fast::c2r_zr_kernel<L, RM, false>, Z::lds_bytes(), Z::C::NT, nblocks, groups, a, s, attr_odd);
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+template <int RM, int... Ls>
+hipError_t c2r_zr_dispatch(int L, int nblocks, int nbatch, const PassArgs& a, hipStream_t s,
+                           std::integer_sequence<int, Ls...>) {
+    hipError_t e = hipErrorInvalidValue;
+    ((L == Ls + kZrunC2rMin ? (e = c2r_zr_L<Ls + kZrunC2rMin, RM>(nblocks, nbatch, a, s), 0) : 0), ...);
+    return e;
+}
+// whether the C2R pass of length 2^L and plan RM reads the z-run layout: its rows per block, else 0
+template <int L, int RM>
+constexpr int c2r_zr_rows_L() {
+    return fast::C2RZrCfg<L, RM>::ok ? fast::C2RCfg<L, RM>::P : 0;
+}
+template <int RM, int... Ls>
+int c2r_zr_rows(int L, std::integer_sequence<int, Ls...>) {
+    int p = 0;
+    ((L == Ls + kZrunC2rMin ? (p = c2r_zr_rows_L<Ls + kZrunC2rMin, RM>(), 0) : 0), ...);
+    return p;
+}
+using zr_c2r_seq = std::make_integer_sequence<int, kZrunC2rMax - kZrunC2rMin + 1>;
+
 template <int L, int IN, int AXIS, int RM, int IG>
 hipError_t strided_LV(int var, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     if (var == 0) return strided_L<L, IN, AXIS, RM, IG, 0>(nblocks, nbatch, a, s);
@@ -171,6 +260,7 @@ template <int RM>
 hipError_t strided_rm(int L, int in_mode, int axis, int var, int nblocks, int nbatch, const PassArgs& a,
                       hipStream_t s) {
     using seq = std::make_integer_sequence<int, kMaxFastLog>;
+    if (in_mode == IN_FILL32H && axis == 2 && a.zrun) return var == 0 ? strided_zr<RM>(L, nblocks, nbatch, a, s) : hipErrorInvalidValue;
     // fill passes: one instantiation per coefficient law (igauss 0 or 1)
     if (in_mode == IN_LOAD) return strided_dispatch<RM, IN_LOAD, 1, 0>(L, var, nblocks, nbatch, a, s, seq{});
     if (in_mode == IN_FILL32H && axis == 2)
@@ -185,6 +275,7 @@ hipError_t strided_rm(int L, int in_mode, int axis, int var, int nblocks, int nb
 template <int RM>
 hipError_t c2r_rm(int L, int in_mode, int nblocks, int nbatch, const PassArgs& a, hipStream_t s) {
     using seq = std::make_integer_sequence<int, kMaxFastLog>;
+    if (in_mode == IN_LOADH && a.zrun) return c2r_zr_dispatch<RM>(L, nblocks, nbatch, a, s, zr_c2r_seq{});
     if (in_mode == IN_LOADH) return c2r_dispatch<RM, IN_LOADH>(L, nblocks, nbatch, a, s, seq{});
     if (in_mode == IN_FILLH) return c2r_dispatch<RM, IN_FILLH>(L, nblocks, nbatch, a, s, seq{});
     return hipErrorInvalidValue;

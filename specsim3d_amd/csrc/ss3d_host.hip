@@ -323,6 +323,7 @@ struct ss3d_context {
     bool pipeline = false;   // SS3D_OPT_PIPELINE: FP32 fill of batch k+1 beside the passes of batch k
     bool fuse = true;        // SS3D_OPT_FUSE: fill of batch k+1 and y pass of batch k in one launch
     bool batch_tune = true;  // SS3D_OPT_BATCH_AUTOTUNE: batch creation autotunes every distinct grid
+    bool zrun = true;        // SS3D_OPT_ZRUN: z-run layout of the FP32 fill and y outputs where it applies
     bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
@@ -1224,6 +1225,39 @@ int enqueue_fp32(ss3d_context* c, Lane& L, ss3d_spectrum* sp, int iseed, int iga
     px.a.out_row = nx;
     px.a.out_bstride = ncrop;
     px.a.tw = wtab(npx >= 2 ? lx - 1 : 0);
+    // z-run layout, in pairs of complex (a float4: kx even and odd) with hp = hc/2 kx pairs:
+    //   fill output  [kx pair][z/8][ky][z mod 8]  (npy rows ky, the cropped nz): the runs a fill
+    //                wave stores for one pair and row lie within 64 KB of each other
+    //   y output     [z/8][kx pair][y][z mod 8]   (the cropped ny): the pairs of a C2R tile's
+    //                rows lie 16 ny bytes apart
+    // The y pass takes tiles (kx pair, z/8) of 16 columns (z mod 8, kx mod 2): a slab of 128-byte
+    // runs per tile, read and written contiguously; the C2R pass reads rows (y, z) a run at a time.
+    const int pc = npx >= 2 ? fast_zrun_c2r_rows(lx - 1, px.rm) : 0;
+    const bool zrun = c->zrun && has_z && has_y && lz == fast_zrun_log() && ly == lz && pz.var == 0 && py.var == 0 &&
+                      hc % 16 == 0 && nz % 8 == 0 && pc > 0 && ny % (pc / 8) == 0 && bz * 8 <= 0x7fffffffLL &&
+                      by * 8 <= 0x7fffffffLL && ncrop * 4 <= 0x7fffffffLL;
+    if (zrun) {
+        const int64_t hp = hc / 2;
+        pz.a.zrun = 1;
+        pz.a.out_sT = 16;                   // per ky
+        pz.a.out_sA = 16 * npy;             // per run of 8 z
+        pz.a.out_sC = 16 * npy * (nz / 8);  // per kx pair
+        pz.a.out_bstride = bz;
+        py.a.n0 = (int)(16 * hp);  // column 16 p + 2 (z mod 8) + kx mod 2 of kx pair p
+        py.a.nthird = nz / 8;
+        py.a.in_sA = 16;
+        py.a.in_sT = 16 * npy;
+        py.a.in_sC = 16 * npy * (nz / 8);
+        py.a.in_bstride = bz;
+        py.a.out_sA = 16;
+        py.a.out_sC = 16 * ny;
+        py.a.out_sT = 16 * ny * hp;
+        py.a.out_bstride = by;
+        px.a.zrun = 1;
+        px.a.in_sA = 16;              // per y
+        px.a.in_sC = 16 * ny;         // per kx pair
+        px.a.in_sT = 16 * ny * hp;    // per run of 8 z
+    }
 
     const bool staged = has_y || has_z;  // a fill pass followed by further passes
     const Fp32Work wk = fp32_work(sp, nx, ny, nz, fill_column_tile(c, sp));
@@ -1629,6 +1663,10 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
     }
     if (option == SS3D_OPT_BATCH_AUTOTUNE) {
         c->batch_tune = value != 0;
+        return SS3D_OK;
+    }
+    if (option == SS3D_OPT_ZRUN) {
+        c->zrun = value != 0;
         return SS3D_OK;
     }
     return fail(SS3D_EINVAL, "unknown option %d", option);

@@ -68,6 +68,7 @@ struct PassArgs {
     int nblocks;      // blocks per batch member
     int nbatch;       // batch members of the launch
     int bpb;          // FP32 fill passes: batch members per block (grid y = ceil(nbatch / bpb))
+    int zrun;         // FP32 z fill and C2R passes: z-run layout of the fill and y outputs (fast_zrun_*)
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of tiles so
@@ -117,6 +118,13 @@ bool fast_same_plan(int L, int rm1, int rm2);  // the FP32 radix plans of length
 bool fast_fusable(int L, int rm, int LY, int rmy);
 hipError_t launch_fast_fused(int L, int rm, int LY, const PassArgs& az, const PassArgs& ay, const FuseMap& m,
                              hipStream_t s);
+// z-run layout (PassArgs::zrun = 1): the z fill of length 2^fast_zrun_log() (tile 16, any plan, tile
+// variant 0; alone or fused with a y pass of that length) writes runs of 8 z per column pair, the
+// y pass reads and writes them through its strides, and the C2R pass of length 2^L and plan rm
+// reads them when fast_zrun_c2r_rows(L, rm) > 0 (its rows per block, a multiple of 8: the 8 z of a
+// run at rows/8 consecutive y)
+int fast_zrun_log();
+int fast_zrun_c2r_rows(int L, int rm);
 
 // FP64 realization passes with compile-time lengths 2^1 .. 2^kMaxFast64Log (ss3d_fast64.h), bit
 // for bit the arithmetic of launch_pass64.  kind 4: x pass (fill + transform of rows, OUT_STORE),

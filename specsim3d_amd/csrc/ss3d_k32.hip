@@ -44,6 +44,18 @@ hipError_t launch_fast_c2r(int L, int in_mode, int rm, int nblocks, int nbatch, 
     }
 }
 
+int fast_zrun_log() { return fastlaunch::kZrunLog; }
+
+int fast_zrun_c2r_rows(int L, int rm) {
+    if (L < fastlaunch::kZrunC2rMin || L > fastlaunch::kZrunC2rMax) return 0;
+    switch (rm) {
+        case 3: return fastlaunch::c2r_zr_rows<3>(L, fastlaunch::zr_c2r_seq{});
+        case 4: return fastlaunch::c2r_zr_rows<4>(L, fastlaunch::zr_c2r_seq{});
+        case 5: return fastlaunch::c2r_zr_rows<5>(L, fastlaunch::zr_c2r_seq{});
+        default: return 0;
+    }
+}
+
 bool fast_same_plan(int L, int rm1, int rm2) {
     auto np = [L](int rm) { return rm == 3 ? fast::npass(L, 3) : rm == 4 ? fast::npass(L, 4) : fast::npass(L, 5); };
     auto rl = [L](int rm, int j) {
