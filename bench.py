@@ -103,7 +103,7 @@ class Regions:
 
 # pass kind of the library's profile -> kernel of that pass (rocprofv3 names)
 KIND_KERNELS = {"real_zy_fp32": r"fill_load_kernel<", "real_z_fp32": r"fast::strided_kernel<\d+, \d+, 3, 2,",
-                "real_y_fp32": r"fast::strided_kernel<\d+, \d+, 2, 1,", "real_x_c2r_fp32": r"fast::c2r_(loop_)?kernel<"}
+                "real_y_fp32": r"fast::strided_kernel<\d+, \d+, 2, 1,", "real_x_c2r_fp32": r"fast::c2r_(loop_|zr_)?kernel<"}
 
 
 def trace_summary(trace_csv, marker_csv, bench_json, command):

@@ -478,6 +478,7 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
         if constexpr (ZR) __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         else __syncthreads();
     };
+
     const int i0base = (tile % a.ntiles0) * P;
     // element offsets of the tile's first column in the input and output (column-tile strides
     // sC: P for x-fastest buffers, the size of a column tile's slab for column-tiled ones)
@@ -491,6 +492,16 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
     // radix passes (LDS exchange between them), then the store of batch member b:
     // butterfly g + G*i, element r at out_pos; positions >= n_out pruned
     auto transform_store = [&](int b, const int g, const int q) __attribute__((always_inline)) {
+        // exchange read of position g + m of the thread's pair (m constant after unrolling): with
+        // z-run slots, a multiple m of EPT adds ppos(m) to ppos(g) (the padding counts whole
+        // groups of EPT positions), so the reads of a pass share one base register
+        const int xbase = ZR ? q * ZSLOT + C::ppos(g) : 0;
+        auto xread = [&](const int m) __attribute__((always_inline)) {
+            if constexpr (ZR) {
+                if (m % EPT == 0) return xbase + C::ppos(m);
+            }
+            return xslot(g + m, q);
+        };
         [&]<int... J>(std::integer_sequence<int, J...>) {
             (
                 [&] {
@@ -512,7 +523,7 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
 #pragma unroll
                             for (int r = 0; r < (1 << LR); ++r) {
                                 const int e = i * (1 << LR) + r;
-                                const float4 w = lds4[xslot(g + G * i + r * (N >> LR), q)];
+                                const float4 w = lds4[xread(i * G + r * (N >> LR))];
                                 v0[e] = mk(w.x, w.y);
                                 v1[e] = mk(w.z, w.w);
                             }
@@ -550,17 +561,18 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
             const uint32_t sZ = (uint32_t)(a.out_sA >> 1), sP = (uint32_t)(a.out_sC >> 1), sK = (uint32_t)(a.out_sT >> 1);
             const __amdgpu_buffer_rsrc_t rs =
                 rsrc(reinterpret_cast<f2*>(a.out) + (int64_t)b * a.out_bstride, (int)(a.out_bstride * 8));
+            static_assert(NSLl >= 3, "the elements of a last-pass butterfly lie whole runs apart");
             const uint32_t o0 = (uint32_t)(i0base / 2 + q) * sP + (uint32_t)third * sK;
+            const uint32_t rstep = (sZ << (NSLl - 3)) * 16u;  // bytes from element r to r + 1 (uniform)
 #pragma unroll
             for (int i = 0; i < NBl; ++i) {
                 const int pos0 = out_pos<L, M - 1, RM>(g + G * i, 0);
                 const int wpos0 = __builtin_amdgcn_readfirstlane(pos0);
+                const uint32_t b0 = (o0 + (uint32_t)(pos0 >> 3) * sZ + (uint32_t)(pos0 & 7)) * 16u;
 #pragma unroll
                 for (int r = 0; r < (1 << LRl); ++r) {
                     const int e = i * (1 << LRl) + r;
-                    const int z = pos0 + (r << NSLl);
-                    const uint32_t off =
-                        active && z < nout ? (o0 + (uint32_t)(z >> 3) * sZ + (uint32_t)(z & 7)) * 16u : kOob;
+                    const uint32_t off = active && pos0 + (r << NSLl) < nout ? b0 + (uint32_t)r * rstep : kOob;
                     if ((!HALF || r < (1 << LRl) / 2) && wpos0 + (r << NSLl) < nout) bstore16(pack2(v0[e], v1[e]), rs, off);
                 }
             }
