@@ -189,6 +189,31 @@ __device__ __forceinline__ f2 coef_signed(float amps, uint32_t st) {
 __device__ __forceinline__ f2 coef(float amp, uint32_t st, int igauss, bool conj, bool selfc) {
     return igauss ? coef_t<1>(amp, st, conj, selfc) : coef_t<0>(amp, st, conj, selfc);
 }
+// coef_signed's value as its parts: the coefficient is (|R| c, R s)
+template <int IG>
+__device__ __forceinline__ void coef_parts(float amps, uint32_t st, float& R, float& c, float& s) {
+    if constexpr (IG == 0) {
+        const float inv_m = 4.6566128752457969e-10f;  // 1/(2^31-1)
+        const float u = (float)st * inv_m;
+        R = amps * 1.2011224087864498f;
+        c = __builtin_amdgcn_cosf(u);
+        s = __builtin_amdgcn_sinf(u);
+    } else {
+        const double d = (double)st;
+        const float u1 = (float)__builtin_amdgcn_fract(d * (1.0 / 2147483647.0));
+        const float u2 = (float)__builtin_amdgcn_fract(d * (16807.0 / 2147483647.0));
+        R = amps * __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(u1));
+        c = __builtin_amdgcn_cosf(u2);
+        s = __builtin_amdgcn_sinf(u2);
+    }
+}
+// a radix-2 stage's sum and difference of the coefficients (|Ra| ca, Ra sa) and (|Rb| cb, Rb sb), the
+// second one's products fused into the sum and difference
+__device__ __forceinline__ void stage1_parts(f2& lo, f2& hi, float Ra, float ca, float sa, float Rb, float cb, float sb) {
+    const float ax = fabsf(Ra) * ca, ay = Ra * sa, bx = fabsf(Rb);
+    lo = mk(__builtin_fmaf(bx, cb, ax), __builtin_fmaf(Rb, sb, ay));
+    hi = mk(__builtin_fmaf(-bx, cb, ax), __builtin_fmaf(-Rb, sb, ay));
+}
 
 // Fill of a half-spectrum column kx (kx <= npx/2), following the walk of
 // src/specsim3d.f90:1305-1333: the site of (kx, ky, kz) is the point itself when its row
@@ -528,7 +553,12 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                                 v1[e] = mk(w.z, w.w);
                             }
                     }
-                    if constexpr (Jc > 0 && C::TWL) {
+                    if constexpr (Jc == 0 && IN == IN_FILL32H) {
+                        // the member loop has done the first radix-2 stage of the single pass-0
+                        // butterfly (radix EPT, no twiddles), in the order of idft_stages
+                        idft_stages<C::EL, 4>(v0);
+                        idft_stages<C::EL, 4>(v1);
+                    } else if constexpr (Jc > 0 && C::TWL) {
                         radix_pass_tan<L, Jc, G, RM>(v0, g, twl);
                         radix_pass_tan<L, Jc, G, RM>(v1, g, twl);
                     } else {
@@ -764,23 +794,48 @@ __device__ __forceinline__ void strided_tile(const PassArgs& a, const int tile, 
                 // keep the site values opaque so that nothing derived from them is hoisted
                 // out of the member loop (registers)
                 asm volatile("" : "+v"(w0[r]), "+v"(w1[r]), "+v"(ampv[r].x), "+v"(ampv[r].y));
-                v0[r] = coef_signed<IG>(ampv[r].x, pm_mul_lazy(w0[r], rb));
-                v1[r] = coef_signed<IG>(ampv[r].y, pm_mul_lazy(w1[r], rb));
             }
+            // The coefficients enter pass 0's single butterfly (radix EPT, no twiddles) through its
+            // first radix-2 stage: elements j and j + EPT/2 give positions brev(j) and brev(j) + 1 of
+            // the order idft_stages works in, their sum and difference, with the second element's
+            // amplitude products fused into them.
+            constexpr int H2 = EPT / 2;
+            auto stage1_col = [&](f2* v, const bool second) __attribute__((always_inline)) {
+#pragma unroll
+                for (int j = 0; j < H2; ++j) {
+                    float Ra, ca, sa, Rb, cb, sb;
+                    coef_parts<IG>(second ? ampv[j].y : ampv[j].x, pm_mul_lazy(second ? w1[j] : w0[j], rb), Ra, ca, sa);
+                    coef_parts<IG>(second ? ampv[j + H2].y : ampv[j + H2].x,
+                                   pm_mul_lazy(second ? w1[j + H2] : w0[j + H2], rb), Rb, cb, sb);
+                    const int k = brev(j, C::EL);
+                    stage1_parts(v[k], v[k + 1], Ra, ca, sa, Rb, cb, sb);
+                }
+            };
+            stage1_col(v1, true);
             if (col0) {  // kx = 0: self-conjugate sites; the Nyquist column (kx = npx/2, its own partner)
                 int zero = 0;  // opaque: the addresses below are not kept across the loop
                 asm volatile("" : "+v"(zero));
                 const float* __restrict__ ab = abase + zero;
+                f2 cv[EPT];
 #pragma unroll
                 for (int r = 0; r < EPT; ++r) {
+                    cv[r] = coef_signed<IG>(ampv[r].x, pm_mul_lazy(w0[r], rb));
                     const uint32_t st = pm_mul(w0[r], rb);  // column factor of kx = 0 is 1
                     const bool cj = (conjm >> r) & 1u, sf = (selfm >> r) & 1u;
-                    if (sf) v0[r] = coef_t<IG>(fabsf(ampv[r].x), st, false, true);
+                    if (sf) cv[r] = coef_t<IG>(fabsf(ampv[r].x), st, false, true);
                     if (pack) {
                         const f2 zn = coef_t<IG>(__ldg(ab + r * astep + kxn), pm_mul(st, txn), cj, sf);
-                        v0[r] = mk(v0[r].x - zn.y, v0[r].y + zn.x);
+                        cv[r] = mk(cv[r].x - zn.y, cv[r].y + zn.x);
                     }
                 }
+#pragma unroll
+                for (int j = 0; j < H2; ++j) {
+                    const int k = brev(j, C::EL);
+                    v0[k] = add(cv[j], cv[j + H2]);
+                    v0[k + 1] = sub(cv[j], cv[j + H2]);
+                }
+            } else {
+                stage1_col(v0, false);
             }
             // per-member recomputation of the thread's LDS, twiddle and store addresses (opaque
             // thread indices) instead of keeping them in registers across the member loop
