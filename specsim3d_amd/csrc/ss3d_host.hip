@@ -1373,16 +1373,21 @@ int batch_size(const ss3d_spectrum* sp, int nx, int ny, int nz, int prec, int nr
                             ? ((int64_t)nx * sp->npy * sp->npz + (int64_t)nx * ny * sp->npz) * 16
                             : ((int64_t)std::max(2, sp->npx / 2) * (sp->npy * (int64_t)nz + (int64_t)ny * nz)) * 8;
     // `per` counts one fill output and one y output per member (the second fill buffer of the
-    // fused launch comes on top).  16 GiB of it (of 288 GB): 8 realizations per launch up to
-    // 1024x1024x512 periodic grids, whose fill pass then amortizes its per-site work over 8
-    // members (63 instead of 54 Gcells/s for 512x512x256 in 1024x1024x512)
+    // fused launch comes on top).  16 GiB of it (of 288 GB): FP32 realizations 16 per launch at
+    // 512^3 periodic grids and 10 at 1024x1024x512, whose fill pass then amortizes its per-site
+    // work over that many members (at 1024x1024x512, 8 members gave 63 against 54 Gcells/s for
+    // one; at 512^3, 16 members per launch run the 256^3 step 2-3% faster than 8); FP64
+    // realizations 8 per launch (10 ran the 64 realizations of the 256^3 step 3% slower: 6 x 10 + 4)
     const int64_t budget = int64_t(16) << 30;
     int64_t nb = std::max<int64_t>(1, budget / std::max<int64_t>(per, 1));
-    // small grids: enough realizations per launch to fill the GPU; large grids: up to 8
+    // small grids: enough realizations per launch to fill the GPU; large grids: up to 16 (FP32) or 8
     const int64_t fill = (int64_t(1) << 24) / std::max<int64_t>((int64_t)sp->npx * sp->npy * sp->npz, 1);
-    nb = std::min<int64_t>(nb, std::max<int64_t>(fill, 8));
-    nb = std::min<int64_t>(nb, nreal);
-    return (int)std::max<int64_t>(nb, 1);
+    nb = std::min<int64_t>(nb, std::max<int64_t>(fill, prec == SS3D_FP64 ? 8 : 16));
+    nb = std::max<int64_t>(std::min<int64_t>(nb, nreal), 1);
+    // the fewest batches of at most nb realizations, as even as possible (32 realizations at most
+    // 10 per launch: 4 x 8 rather than 10, 10, 10, 2)
+    const int64_t nbat = (std::max(nreal, 1) + nb - 1) / nb;
+    return (int)((std::max(nreal, 1) + nbat - 1) / nbat);
 }
 
 }  // namespace

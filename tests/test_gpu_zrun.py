@@ -28,7 +28,7 @@ CASES = [
     (BENCH, 0.3, (256, 256, 256), 17, 1, 3, 5, 0),
     (SMALLX, 0.1, (128, 248, 256), 9, 0, None, None, 0),
     (SMALLX, 0.1, (127, 256, 200), 3, 1, 4, 3, 0),
-    (SMALLX, 0.0, (130, 248, 264), 12, 1, 3, 4, 0),
+    (SMALLX, 0.0, (130, 248, 264), 20, 1, 3, 4, 0),
     (SMALLX, 0.2, (128, 256, 256), 20, 0, 3, 5, 1),
 ]
 
