@@ -6,7 +6,7 @@ arithmetic as in the row layouts, so the fields equal those of SS3D_OPT_ZRUN = 0
 statistics' block sums group other rows (a run of 8 z instead of consecutive y), so mean and
 variance agree to rounding.  The cases cover both coefficient laws, the fused schedule (several
 batches) and the separate fill and y launches (one batch), even and odd cropped nx (8-byte or
-4-byte output stores), the three C2R plans (8 or 16 rows per block), the two-stream pipeline and
+4-byte output stores), the three C2R plans (16 or 32 rows per block), the two-stream pipeline and
 the 256^3 benchmark grid.
 """
 import sys
