@@ -14,7 +14,11 @@
 //   - the x pass is a real-output (C2R) transform computed as an n/2-point complex FFT of
 //     Z[k] = E[k] + i O[k], E = X[k] + conj X[n/2-k], O = (X[k] - conj X[n/2-k]) W_n^k, with
 //     x[2m] = Re z[m], x[2m+1] = Im z[m].  The imaginary parts of X[0] and X[n/2] are dropped,
-//     which is exactly what taking the real part of a complex-to-complex transform does.
+//     which is exactly what taking the real part of a complex-to-complex transform does;
+//   - z-run layout (z and y axes of 2^9): the z fill holds whole column pairs per wave, so its
+//     exchanges need no block barrier, and writes runs of 8 z per kx pair; the y pass reads and
+//     writes those runs through its strides, and a C2R variant moves them to rows in LDS.  Every
+//     column and row is transformed by the same arithmetic as in the row layouts.
 //
 // Stockham pass j (radix R = R_j, Ns = R_0 ... R_{j-1}) on butterfly b in [0, n/R):
 //     v[r] = in[b + r n/R] * W_{Ns R}^{r (b mod Ns)},  v = DFT_R(v),
