@@ -20,16 +20,19 @@ pytestmark = pytest.mark.gpu
 
 BENCH = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]  # exponential sill 0.7, ranges 80/40/10
 SMALLX = [(1, 1.0, 10.0, 60.0, 60.0, 20.0, 0.0, 0.0)]  # short x range: a 256-point x axis
+THIN = [(1, 1.0, 200.0, 60.0, 200.0, 0.0, 0.0, 0.0)]  # long y and z ranges: thin crops in 256x512x512
 
 # (structures, nugget, output grid, realizations, igauss, z/y radix plan, C2R radix plan, pipeline);
 # plans None: the untuned default (largest radix 2^5).  Periodic grids 512^3 (the benchmark) and
-# 256x512x512; nz 264 keeps more than half the z axis (no pruned last pass)
+# 256x512x512; nz 264 keeps more than half the z axis (no pruned last pass); 100x32x16 keeps two
+# runs of z and one C2R tile of y
 CASES = [
     (BENCH, 0.3, (256, 256, 256), 17, 1, 3, 5, 0),
     (SMALLX, 0.1, (128, 248, 256), 9, 0, None, None, 0),
     (SMALLX, 0.1, (127, 256, 200), 3, 1, 4, 3, 0),
     (SMALLX, 0.0, (130, 248, 264), 20, 1, 3, 4, 0),
     (SMALLX, 0.2, (128, 256, 256), 20, 0, 3, 5, 1),
+    (THIN, 0.1, (100, 32, 16), 20, 1, 3, 5, 0),
 ]
 
 

@@ -1,7 +1,8 @@
 """ss3d_realize_device_noalloc (the realization path with no device allocation) against
 ss3d_realize_device, through the C ABI: fields and statistics bit for bit, FP32 and FP64, both
 coefficient laws, two calls in a row (the second on a smaller grid and fewer realizations, so the
-scratch array keeps the first call's contents), the statistics-free form, and the error paths
+scratch array keeps the first call's contents), the z-run layout of the FP32 passes, the
+statistics-free form, and the error paths
 (a grid whose one realization exceeds the scratch array; random-stream tables not built)."""
 import sys
 
@@ -55,6 +56,27 @@ def test_noalloc_equals_allocating_path(eng, prec, igauss):
         sp = eng.spectrum(sts, nug, 1, 1, 1, *npg)
         try:
             (fa, sa), (fb, sb) = run_both(eng, sp, 4242 + k, igauss, 3, nreal, grid, prec)
+        finally:
+            sp.close()
+        assert np.array_equal(fa, fb), np.abs(fa - fb).max()
+        assert np.array_equal(sa, sb)
+
+
+# crops that keep few rows of 256x512x512 periodic grids (z and y axes of 2^9): the FP32 passes run
+# in the z-run layout (wave-local fill, runs of 8 z per kx pair) inside the scratch array's windows;
+# two calls in a row, the second smaller
+ZRUN_STS = [(1, 1.0, 200.0, 60.0, 200.0, 0.0, 0.0, 0.0)]
+ZRUN_CASES = [((100, 32, 16), 20), ((130, 20, 24), 9)]
+
+
+def test_noalloc_zrun_layout(eng):
+    for k, (grid, nreal) in enumerate(ZRUN_CASES):
+        npg = S.periodic_grid(ZRUN_STS, 1, 1, 1, *grid, 1)
+        assert npg == (256, 512, 512), npg
+        say(f"[noalloc] z-run layout, call {k + 1} grid {grid} periodic {npg} nreal {nreal}")
+        sp = eng.spectrum(ZRUN_STS, 0.1, 1, 1, 1, *npg)
+        try:
+            (fa, sa), (fb, sb) = run_both(eng, sp, 777 + k, 1, 2, nreal, grid, S.FP32)
         finally:
             sp.close()
         assert np.array_equal(fa, fb), np.abs(fa - fb).max()
