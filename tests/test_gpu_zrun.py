@@ -6,8 +6,8 @@ arithmetic as in the row layouts, so the fields equal those of SS3D_OPT_ZRUN = 0
 statistics' block sums group other rows (a run of 8 z instead of consecutive y), so mean and
 variance agree to rounding.  The cases cover both coefficient laws, the fused schedule (several
 batches) and the separate fill and y launches (one batch), even and odd cropped nx (8-byte or
-4-byte output stores), the three C2R plans (16 or 32 rows per block), the two-stream pipeline and
-the 256^3 benchmark grid.
+4-byte output stores), the C2R plans and lengths that hold whole runs (8 to 32 rows per block,
+x axes of 256 to 2048 points), the two-stream pipeline and the 256^3 benchmark grid.
 """
 import sys
 
@@ -21,11 +21,13 @@ pytestmark = pytest.mark.gpu
 BENCH = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]  # exponential sill 0.7, ranges 80/40/10
 SMALLX = [(1, 1.0, 10.0, 60.0, 60.0, 20.0, 0.0, 0.0)]  # short x range: a 256-point x axis
 THIN = [(1, 1.0, 200.0, 60.0, 200.0, 0.0, 0.0, 0.0)]  # long y and z ranges: thin crops in 256x512x512
+WIDE = [(1, 1.0, 200.0, 400.0, 200.0, 0.0, 0.0, 0.0)]  # and a long x range: 1024- and 2048-point x axes
 
 # (structures, nugget, output grid, realizations, igauss, z/y radix plan, C2R radix plan, pipeline);
 # plans None: the untuned default (largest radix 2^5).  Periodic grids 512^3 (the benchmark) and
 # 256x512x512; nz 264 keeps more than half the z axis (no pruned last pass); 100x32x16 keeps two
-# runs of z and one C2R tile of y
+# runs of z and one C2R tile of y; 300x32x16 and 1001x20x16 in 1024x512x512 and 2048x512x512 run
+# the C2R pass at 2^9 and 2^10 (plan 5: 16 and 8 rows per block)
 CASES = [
     (BENCH, 0.3, (256, 256, 256), 17, 1, 3, 5, 0),
     (SMALLX, 0.1, (128, 248, 256), 9, 0, None, None, 0),
@@ -33,6 +35,8 @@ CASES = [
     (SMALLX, 0.0, (130, 248, 264), 20, 1, 3, 4, 0),
     (SMALLX, 0.2, (128, 256, 256), 20, 0, 3, 5, 1),
     (THIN, 0.1, (100, 32, 16), 20, 1, 3, 5, 0),
+    (WIDE, 0.1, (300, 32, 16), 20, 0, 3, 5, 0),
+    (WIDE, 0.0, (1001, 20, 16), 18, 1, 4, 5, 0),
 ]
 
 
@@ -51,7 +55,7 @@ def eng():
 def test_zrun_equals_row_layout(eng, sts, nugget, grid, nreal, igauss, rmzy, rmx, pipe):
     nx, ny, nz = grid
     npg = S.periodic_grid(sts, 1.0, 1.0, 1.0, nx, ny, nz, 1)
-    assert npg[1] == npg[2] == 512 and npg[0] in (256, 512), npg
+    assert npg[1] == npg[2] == 512 and npg[0] in (256, 512, 1024, 2048), npg
     say(f"[zrun] grid {grid} periodic {npg} nreal {nreal} igauss {igauss} plans {rmzy}/{rmx} pipeline {pipe}")
     sp = eng.spectrum(sts, nugget, 1.0, 1.0, 1.0, *npg)
     lx = int(np.log2(npg[0])) - 1
