@@ -104,7 +104,8 @@ int64_t ss3d_realization_state(int iseed, int npx, int npy, int npz, int igauss,
  * spectrum of ctx from such a file; with st != NULL the file must have been made for exactly that
  * model, spacing and grid (SS3D_EINVAL otherwise); st = NULL takes the file's key.  The payload is
  * checked against the diagnostics (sum of amp^2 = positive spectral mass).  Realizations of a loaded
- * spectrum are identical to those of the spectrum that was saved. */
+ * spectrum are identical to those of the spectrum that was saved.  Spectra without FP64 amplitudes
+ * (SS3D_OPT_SPEC_FP32) are not saved (SS3D_EINVAL). */
 int ss3d_spectrum_save(const ss3d_spectrum* spec, const char* path);
 int ss3d_spectrum_load(ss3d_context* ctx, const char* path, const ss3d_structure* st, int nst, double nugget,
                        double dx, double dy, double dz, int npx, int npy, int npz, ss3d_spectrum** out,
@@ -196,6 +197,15 @@ int ss3d_tuning_set_tile(ss3d_context* ctx, int kind, int logn, int rm, int vari
  * 0 keeps the row layouts.  Same field values bit for bit (the statistics' block sums group other
  * rows, so they may differ in the last bits). */
 #define SS3D_OPT_ZRUN 5
+/* SS3D_OPT_SPEC_FP32 (default 0): ss3d_spectrum_create builds spectra for FP32 realizations only.
+ * No FP64 amplitudes are kept (FP64 realizations from such a spectrum fail with SS3D_EINVAL), and
+ * the transforms after the x pass run over the columns kx <= npx/2 alone: the spectral density of
+ * the real covariance grid is Hermitian, S(-k) = conj S(k), and the FP32 amplitudes use no other
+ * column.
+ * The FP32 amplitudes equal those of a full build bit for bit; the diagnostics agree to rounding
+ * (their sums count the columns 0 < kx < npx/2 twice).  The request loop and ss3d_batch_create
+ * build the spectra of their FP32-only work this way whatever the option. */
+#define SS3D_OPT_SPEC_FP32 6
 int ss3d_context_set_option(ss3d_context* ctx, int option, int value);
 
 /* ---------------------------------------------------------------- request loop

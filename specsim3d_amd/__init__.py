@@ -25,6 +25,7 @@ OPT_FP64_GENERIC = 2
 OPT_FUSE = 3
 OPT_BATCH_AUTOTUNE = 4
 OPT_ZRUN = 5
+OPT_SPEC_FP32 = 6
 SCRATCH_BYTES = 256 << 20  # SS3D_SCRATCH_BYTES: the scratch array of ss3d_realize_device_noalloc
 
 ERRORS = {1: "invalid argument", 2: "too big", 3: "out of memory", 4: "HIP error", 5: "no device", 6: "I/O error"}

@@ -173,6 +173,8 @@ __global__ __launch_bounds__((1 << L) / (1 << el(L)) * strided_P(L)) void stride
         // operations of the generic kernel's OUT_SPEC stage
         const double rn = (double)a.npx * (double)a.npy * (double)a.npz;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        // columns kx <= npx/2 alone: column kx stands for itself and its mirror npx - kx
+        const bool twice = a.spec_half && i0 != 0 && 2 * i0 != a.npx;
         if (active) {
 #pragma unroll
             for (int i = 0; i < NBl; ++i)
@@ -185,11 +187,12 @@ __global__ __launch_bounds__((1 << L) / (1 << el(L)) * strided_P(L)) void stride
                     acc[0] = fmax(acc[0], fabs(si));
                     acc[1] = fmax(acc[1], fabs(sr));
                     double amp;
+                    const double sw = twice ? sr + sr : sr;
                     if (sr < 0.0) {
-                        acc[2] += sr;
+                        acc[2] += sw;
                         amp = 0.0;
                     } else {
-                        acc[3] += sr;
+                        acc[3] += sw;
                         amp = sqrt(sr);
                     }
                     const int64_t row = j + (int64_t)a.npy * k;

@@ -324,6 +324,7 @@ struct ss3d_context {
     bool fuse = true;        // SS3D_OPT_FUSE: fill of batch k+1 and y pass of batch k in one launch
     bool batch_tune = true;  // SS3D_OPT_BATCH_AUTOTUNE: batch creation autotunes every distinct grid
     bool zrun = true;        // SS3D_OPT_ZRUN: z-run layout of the FP32 fill and y outputs where it applies
+    bool spec32 = false;     // SS3D_OPT_SPEC_FP32: ss3d_spectrum_create builds FP32-only spectra
     bool fp64_generic = false;  // SS3D_OPT_FP64_GENERIC: FP64 realizations through the generic pass kernel
     int target_e64 = 4096;   // elements per FP64 tile
     int target_e32 = 8192;   // elements per FP32 tile
@@ -839,15 +840,23 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     const bool f64ok = !c->fp64_generic;
     auto fits = [](int l) { return l >= 1 && l <= kMaxFast64Log; };
     if (f64ok && (has_y || has_z) && fits(lx)) px.fast = 6;
+    // FP32-only spectra: the density of a real covariance grid is Hermitian, S(-k) = conj S(k), and the
+    // FP32 amplitudes hold the columns kx <= npx/2, so the passes after the x pass transform those
+    // columns alone (each column by the same operations as in a full build) and the OUT_SPEC sums count
+    // the columns 0 < kx < npx/2 twice (Re S(-k) = Re S(k))
+    const bool half = !with64 && px.fast == 6 && (!has_y || fits(ly)) && (!has_z || fits(lz)) && sp->hx < npx;
+    const int ncol = half ? sp->hx : npx;
+    if (half) px.a.n_out = ncol;
     SS3D_TRY(run_pass(c, px, 1, L.stream, L.partial));
     Pass* last = &px;
     Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a, 1};
     if (has_y) {
         py.a.axis = 1;
         py.a.logn = ly;
-        py.a.n0 = npx;
+        py.a.n0 = ncol;
         py.a.nthird = npz;
         py.a.n_out = npy;
+        py.a.spec_half = half && !has_z;
         py.a.in = L.wa.p;
         py.a.out = L.wa.p;
         py.a.in_sA = py.a.out_sA = npx;
@@ -861,9 +870,10 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     if (has_z) {
         pz.a.axis = 2;
         pz.a.logn = lz;
-        pz.a.n0 = npx;
+        pz.a.n0 = ncol;
         pz.a.nthird = npy;
         pz.a.n_out = npz;
+        pz.a.spec_half = half;
         pz.a.in = L.wa.p;
         pz.a.out = L.wa.p;
         pz.a.in_sA = pz.a.out_sA = (int64_t)npx * npy;
@@ -896,7 +906,7 @@ int ss3d_spectrum_create(ss3d_context* c, const ss3d_structure* st, int nst, dou
     if (!c || !out) return fail(SS3D_EINVAL, "null argument");
     *out = nullptr;
     std::unique_ptr<ss3d_spectrum> sp;
-    SS3D_TRY(spectrum_build(c, c->main, st, nst, nugget, dx, dy, dz, npx, npy, npz, sp));
+    SS3D_TRY(spectrum_build(c, c->main, st, nst, nugget, dx, dy, dz, npx, npy, npz, sp, nullptr, !c->spec32));
     SS3D_TRY(spectrum_read_diag(sp.get(), c->main.stream));
     if (diag) *diag = sp->diag;
     sp->key_nst = nst;
@@ -1672,6 +1682,10 @@ int ss3d_context_set_option(ss3d_context* c, int option, int value) {
     }
     if (option == SS3D_OPT_ZRUN) {
         c->zrun = value != 0;
+        return SS3D_OK;
+    }
+    if (option == SS3D_OPT_SPEC_FP32) {
+        c->spec32 = value != 0;
         return SS3D_OK;
     }
     return fail(SS3D_EINVAL, "unknown option %d", option);

@@ -69,6 +69,7 @@ struct PassArgs {
     int nbatch;       // batch members of the launch
     int bpb;          // FP32 fill passes: batch members per block (grid y = ceil(nbatch / bpb))
     int zrun;         // FP32 z fill and C2R passes: z-run layout of the fill and y outputs (fast_zrun_*)
+    int spec_half;    // last spectrum pass over the columns kx <= npx/2 alone: sums count 0 < kx < npx/2 twice
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of tiles so

@@ -119,10 +119,10 @@ __global__ __launch_bounds__(256) void stats_finalize_kernel(const double* __res
 }
 
 // (max|Im S|, max|Re S|, sneg, spos) over the spectrum blocks (f90:1277-1288).
-__global__ __launch_bounds__(256) void diag_finalize_kernel(const double* __restrict__ partial, int nblocks,
-                                                            double* __restrict__ diag4) {
+__global__ __launch_bounds__(1024) void diag_finalize_kernel(const double* __restrict__ partial, int nblocks,
+                                                             double* __restrict__ diag4) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < nblocks; i += 256) {
+    for (int i = threadIdx.x; i < nblocks; i += 1024) {
         v[0] = fmax(v[0], partial[4 * i]);
         v[1] = fmax(v[1], partial[4 * i + 1]);
         v[2] += partial[4 * i + 2];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void diag_finalize_kernel(const double* __rest
     }
     const int ops[4] = {1, 1, 0, 0};
     double out[4];
-    block_reduce_write<256, 4>(v, ops, out);
+    block_reduce_write<1024, 4>(v, ops, out);
     if (threadIdx.x == 0)
         for (int c = 0; c < 4; ++c) diag4[c] = out[c];
 }
@@ -142,7 +142,7 @@ hipError_t launch_stats_finalize(const double* partial, int nblocks, int nbatch,
 }
 
 hipError_t launch_diag_finalize(const double* partial, int nblocks, double* diag4, hipStream_t s) {
-    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(256), 0, s, partial, nblocks, diag4);
+    hipLaunchKernelGGL(diag_finalize_kernel, dim3(1), dim3(1024), 0, s, partial, nblocks, diag4);
     return hipGetLastError();
 }
 
