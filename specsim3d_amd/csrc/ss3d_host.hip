@@ -846,7 +846,11 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     // the columns 0 < kx < npx/2 twice (Re S(-k) = Re S(k))
     const bool half = !with64 && px.fast == 6 && (!has_y || fits(ly)) && (!has_z || fits(lz)) && sp->hx < npx;
     const int ncol = half ? sp->hx : npx;
+    // row stride of W: the kept columns rounded up to whole 128-byte column tiles (z planes of about half
+    // the size for the z pass to walk)
+    const int64_t ws = half ? (ncol + 7) & ~7 : npx;
     if (half) px.a.n_out = ncol;
+    px.a.out_row = ws;
     SS3D_TRY(run_pass(c, px, 1, L.stream, L.partial));
     Pass* last = &px;
     Pass py{64, IN_LOAD, has_z ? OUT_STORE : OUT_SPEC, strided_shape(ly, npx, 128, 16, c->target_e64, emax), 0, a, 1};
@@ -859,8 +863,8 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
         py.a.spec_half = half && !has_z;
         py.a.in = L.wa.p;
         py.a.out = L.wa.p;
-        py.a.in_sA = py.a.out_sA = npx;
-        py.a.in_sT = py.a.out_sT = (int64_t)npx * npy;
+        py.a.in_sA = py.a.out_sA = ws;
+        py.a.in_sT = py.a.out_sT = ws * npy;
         py.a.tw = tw_ptr(c->tw64f, ly, 16);
         if (f64ok && fits(ly)) py.fast = 5;
         SS3D_TRY(run_pass(c, py, 1, L.stream, L.partial));
@@ -876,8 +880,8 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
         pz.a.spec_half = half;
         pz.a.in = L.wa.p;
         pz.a.out = L.wa.p;
-        pz.a.in_sA = pz.a.out_sA = (int64_t)npx * npy;
-        pz.a.in_sT = pz.a.out_sT = npx;
+        pz.a.in_sA = pz.a.out_sA = ws * npy;
+        pz.a.in_sT = pz.a.out_sT = ws;
         pz.a.tw = tw_ptr(c->tw64f, lz, 16);
         if (f64ok && fits(lz)) pz.fast = 5;
         SS3D_TRY(run_pass(c, pz, 1, L.stream, L.partial));
