@@ -753,6 +753,9 @@ def main():
                     help="seconds of arrivals at 200 req/s (the overload run)")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving legs and print their JSON (not the benchmark line)")
+    ap.add_argument("--spectrum-only", type=int, default=0, metavar="REPS",
+                    help="build the main workload's spectral density REPS times in full and REPS times FP32-only, "
+                         "print the wall times as JSON and exit (for kernel traces of the spectrum passes)")
     ap.add_argument("--trace-summary", default="",
                     help="summarize a rocprofv3 kernel_trace.csv of this script (no GPU work) and exit")
     ap.add_argument("--marker-csv", default="", help="with --trace-summary: the run's marker_api_trace.csv")
@@ -811,6 +814,23 @@ def main():
         eng.set_option(S.OPT_FUSE, args.fuse)
     if args.zrun >= 0:
         eng.set_option(S.OPT_ZRUN, args.zrun)
+    if args.spectrum_only > 0:
+        grid = S.periodic_grid(STRUCTS, 1.0, 1.0, 1.0, *GRID, 1)
+        eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *grid).close()  # loads the kernels
+        times = {}
+        try:
+            for name, opt in (("full", 0), ("fp32_only", 1)):
+                eng.set_option(S.OPT_SPEC_FP32, opt)
+                times[name] = []
+                for _ in range(args.spectrum_only):
+                    t0 = time.perf_counter()
+                    eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *grid).close()
+                    times[name].append(round((time.perf_counter() - t0) * 1e3, 3))
+        finally:
+            eng.set_option(S.OPT_SPEC_FP32, 0)
+        eng.close()
+        print(json.dumps({"spectrum_build_ms": times, "periodic_grid": grid}), flush=True)
+        return
     if args.serve_only:
         # the radix plans the full run would have tuned for this grid
         spec = eng.spectrum(STRUCTS, NUGGET, 1.0, 1.0, 1.0, *S.periodic_grid(STRUCTS, 1.0, 1.0, 1.0, *GRID, 1))
