@@ -847,8 +847,8 @@ static int spectrum_build(ss3d_context* c, Lane& L, const ss3d_structure* st, in
     const bool half = !with64 && px.fast == 6 && (!has_y || fits(ly)) && (!has_z || fits(lz)) && sp->hx < npx;
     const int ncol = half ? sp->hx : npx;
     // row stride of W: the kept columns rounded up to whole 128-byte column tiles (z planes of about half
-    // the size for the z pass to walk)
-    const int64_t ws = half ? (ncol + 7) & ~7 : npx;
+    // the size for the z pass to walk), at most npx (W holds npx npy npz values)
+    const int64_t ws = half ? std::min((ncol + 7) & ~7, npx) : npx;
     if (half) px.a.n_out = ncol;
     px.a.out_row = ws;
     SS3D_TRY(run_pass(c, px, 1, L.stream, L.partial));

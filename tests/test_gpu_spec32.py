@@ -5,8 +5,8 @@ density of a real covariance is Hermitian, S(-k) = conj S(k): the maxima of |Re 
 kept columns are those of the whole grid, and the sums count the mirrored columns twice, so the
 diagnostics agree with a full build's to rounding and sum(S) = C(0) still holds.  FP64 realizations and saving need the FP64 amplitudes and are refused.
 Periodic grids: 512^3 (the benchmark), a 3-D Gaussian-model grid, npz = 1 (the y pass is the last one),
-npy = 1 (x then z), npx = 2 (no column to leave out), a 1-D grid (x pass alone) and the generic FP64
-passes (full builds)."""
+npy = 1 (x then z), npx = 2 (no column to leave out), npx = 4 and 32 (kept columns 3 and 17: rows of 4
+and 24 values), a 1-D grid (x pass alone) and the generic FP64 passes (full builds)."""
 import sys
 
 import numpy as np
@@ -27,6 +27,8 @@ CASES = [
     (SPH, 0.0, (64, 16, 1), (128, 128, 1), 4, 1, 0),
     (SPH, 0.2, (40, 1, 20), (128, 1, 64), 3, 1, 0),
     (SPH, 0.1, (2, 8, 8), (2, 16, 16), 2, 1, 0),
+    (SPH, 0.1, (3, 8, 8), (4, 16, 16), 2, 0, 0),
+    (GAUSS, 0.2, (20, 12, 10), (32, 16, 16), 3, 1, 0),
     (BENCH, 0.1, (40, 1, 1), (256, 1, 1), 2, 0, 0),
     (GAUSS, 0.0, (24, 20, 12), (64, 64, 32), 2, 1, 1),
 ]
