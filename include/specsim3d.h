@@ -59,8 +59,9 @@ typedef struct ss3d_spectrum ss3d_spectrum;
 
 /* ABI version, major * 10000 + minor * 100: 1.1 adds the output writer, spectrum files and
  * conditioning to 1.0 (grid sizing, spectra, realizations, variograms, request loop, batch graph);
- * 1.2 adds the allocation-free realization entry point and the tile entries of the tuning table. */
-#define SS3D_VERSION 10200
+ * 1.2 adds the allocation-free realization entry point and the tile entries of the tuning table;
+ * 1.3 adds the options SS3D_OPT_ZRUN and SS3D_OPT_SPEC_FP32. */
+#define SS3D_VERSION 10300
 int ss3d_version(void);
 
 /* Copies the last error message of the calling thread (NUL-terminated, truncated to len). */
