@@ -3,10 +3,12 @@ after the x pass transform the columns kx <= npx/2 alone, by the same operations
 the FP32 amplitudes -- and every FP32 field drawn from them -- equal a full build's bit for bit.  The
 density of a real covariance is Hermitian, S(-k) = conj S(k): the maxima of |Re S| and |Im S| over the
 kept columns are those of the whole grid, and the sums count the mirrored columns twice, so the
-diagnostics agree with a full build's to rounding and sum(S) = C(0) still holds.  FP64 realizations and saving need the FP64 amplitudes and are refused.
-Periodic grids: 512^3 (the benchmark), a 3-D Gaussian-model grid, npz = 1 (the y pass is the last one),
-npy = 1 (x then z), npx = 2 (no column to leave out), npx = 4 and 32 (kept columns 3 and 17: rows of 4
-and 24 values), a 1-D grid (x pass alone) and the generic FP64 passes (full builds)."""
+diagnostics agree with a full build's to rounding and sum(S) = C(0) still holds.  FP64 realizations
+and saving need the FP64 amplitudes and are refused.
+Periodic grids: 512^3 (the benchmark), 1024x1024x512 (the large job: x axis 2^10, 513 kept columns),
+a 3-D Gaussian-model grid, npz = 1 (the y pass is the last one), npy = 1 (x then z), npx = 2 (no
+column to leave out), npx = 4 and 32 (kept columns 3 and 17: rows of 4 and 24 values), a 1-D grid
+(x pass alone) and the generic FP64 passes (full builds)."""
 import sys
 
 import numpy as np
@@ -19,10 +21,12 @@ pytestmark = pytest.mark.gpu
 BENCH = [(2, 0.7, 80.0, 40.0, 10.0, 30.0, 15.0, 0.0)]
 GAUSS = [(3, 0.9, 12.0, 8.0, 5.0, 120.0, -20.0, 0.0)]
 SPH = [(1, 0.8, 14.0, 9.0, 5.0, 30.0, 15.0, 0.0), (2, 0.1, 20.0, 20.0, 20.0, 0.0, 0.0, 0.0)]
+LARGE = [(3, 1.0, 150.0, 60.0, 20.0, 120.0, -20.0, 0.0)]  # the 1024x1024x512 large-job model
 
 # (structures, nugget, output grid, periodic grid, realizations, igauss, generic FP64 passes)
 CASES = [
     (BENCH, 0.3, (256, 256, 256), (512, 512, 512), 2, 1, 0),
+    (LARGE, 0.0, (512, 512, 256), (1024, 1024, 512), 1, 1, 0),
     (GAUSS, 0.1, (40, 30, 20), (64, 64, 32), 3, 0, 0),
     (SPH, 0.0, (64, 16, 1), (128, 128, 1), 4, 1, 0),
     (SPH, 0.2, (40, 1, 20), (128, 1, 64), 3, 1, 0),
