@@ -442,7 +442,7 @@ def serve(S, eng, rate, duration, seed, nreal=64, nslots=2, spot_frac=0.05, repe
             "spot_checks": len(spots), "spot_max_rel_rms": max(spots) if spots else None}
 
 
-def uneven_batch(S, eng, seed, njobs=32, nlanes=4, reps=5):
+def uneven_batch(S, eng, seed, njobs=32, nlanes=8, reps=5):
     """32 FP32 jobs of uneven grids (nx, ny in {32,64,128,256}, nz in {1,16,64,128}; spherical,
     exponential or gaussian model, ranges U(5,100) per axis, azimuth U(0,180), dip U(-30,30),
     nugget U(0,0.3), sill 1 - nugget, nreal U{1..16}, ipad=1) captured into one HIP graph on
