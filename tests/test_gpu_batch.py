@@ -43,7 +43,7 @@ def alone(eng, job):
     return f, mean, var
 
 
-@pytest.mark.parametrize("nlanes", [1, 4])
+@pytest.mark.parametrize("nlanes", [1, 4, 8])
 def test_graph_replays_match_eager_and_alone(nlanes):
     say(f"[batch] nlanes={nlanes} create")
     with S.Engine(0) as eng:
